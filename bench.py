@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: 3D segment x group integrations/s of the plain transport sweep.
+
+Workload (one GPU): the plain sweep at full shape -- one domain of a
+17x17-assembly core, 400 cm height split into 20 axial domains (domain
+20 cm), 20 coarse x 5 fine axial intervals, 64 azimuthal x 10 polar angles,
+128 energy groups, 5000 2D source regions per assembly, radial ray spacing
+0.05 cm, axial ray spacing 0.25 cm; 2D segments per track Poisson(40),
+lengths U(0.01, 0.5) cm, sigma_t U(0.1, 2.0) per region and group, other
+fields U(0,1); seeded synthetic data.
+
+A step is one power iteration in the reference's call order
+(src/main.c:57-92): transport sweep, renormalisation, source update, k_eff.
+`value` = total integrations (3D segment pieces x groups) of the timed
+steps / wall time of those steps, inputs already resident in HBM.
+
+Extra objects on the JSON line:
+  roofline:     the attenuation kernel's algorithmic bytes per launch,
+                24 B per (piece, group) + 8 B per (track, group), over its
+                HIP-event time, against 8 TB/s HBM peak; `traffic` from the
+                committed rocprofv3 counter summary of the same workload.
+  cpu_baseline: the reference's own OpenMP sweep (reference sources built
+                with the reference Makefile flags, oracle/_ref) timed on this
+                host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+WORKLOAD = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2, radial_ray_sep=0.05,
+                axial_z_sep=0.25, n_azimuthal=64, n_polar_angles=10, n_egroups=128, decompose=1,
+                decomp_assemblies_ax=20, segments_per_track=40, assembly_width=21.42,
+                height=400.0, precision=0.01, n_2D_source_regions_per_assembly=5000)
+SEED = 2
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "attenuate_traffic_plain_sweep_full.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(prob, target_s: float, threads: int):
+    """Reference OpenMP sweep on the first 2D tracks of the same problem."""
+    from oracle import pyoracle as po
+    if not os.path.exists(po.REF_OMP_LIB):
+        return None
+    L = C.CDLL(po.REF_OMP_LIB)
+
+    class RefSample(C.Structure):
+        _fields_ = [("ntracks_2D", C.c_long), ("n_polar", C.c_int), ("z_stacked", C.c_int),
+                    ("G", C.c_int), ("nsr", C.c_long), ("fai", C.c_int), ("cai", C.c_int),
+                    ("decomp", C.c_int), ("axial_exp", C.c_int), ("height", C.c_float),
+                    ("axial_z_sep", C.c_float), ("precision", C.c_float), ("n_mat", C.c_long),
+                    ("az_weight", C.c_void_p), ("seg_offset", C.c_void_p),
+                    ("seg_length", C.c_void_p), ("polar_angle", C.c_void_p),
+                    ("p_weight", C.c_void_p), ("z_height", C.c_void_p),
+                    ("fine_source", C.c_void_p), ("sigT", C.c_void_p), ("vol", C.c_void_p),
+                    ("material", C.c_void_p), ("xs", C.c_void_p), ("scatter", C.c_void_p)]
+
+    L.ref_time_sweep.argtypes = [C.POINTER(RefSample), C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_long)]
+    L.ref_time_sweep.restype = C.c_int
+    a = prob.arrays
+    d = prob.dims
+    per2d = d["n_polar"] * d["z_stacked"]
+
+    def run(n2):
+        s = RefSample()
+        s.ntracks_2D, s.n_polar, s.z_stacked, s.G = n2, d["n_polar"], d["z_stacked"], d["G"]
+        s.nsr, s.fai, s.cai, s.decomp, s.axial_exp = d["nsr"], d["fai"], d["cai"], d["decomp"], d["axial_exp"]
+        s.height, s.axial_z_sep, s.precision, s.n_mat = d["height"], d["axial_z_sep"], 0.01, d["n_mat"]
+        keep = []
+        for k in ("az_weight", "seg_offset", "seg_length", "polar_angle", "fine_source", "sigT",
+                  "vol", "material", "xs", "scatter"):
+            arr = a[k]
+            keep.append(arr)
+            setattr(s, k, arr.ctypes.data)
+        pw = np.ascontiguousarray(a["p_weight"][: n2 * per2d])
+        zh = np.ascontiguousarray(a["z_height"][: n2 * per2d])
+        keep += [pw, zh]
+        s.p_weight, s.z_height = pw.ctypes.data, zh.ctypes.data
+        sec, segs = C.c_double(), C.c_long()
+        rc = L.ref_time_sweep(C.byref(s), threads, C.byref(sec), C.byref(segs))
+        if rc != 0:
+            raise RuntimeError("reference sweep failed")
+        return sec.value, segs.value
+
+    # calibrate on a small sample, then size the sample for ~target_s
+    n_cal = 64
+    t0, s0 = run(n_cal)
+    rate = s0 * d["G"] / max(t0, 1e-9)
+    n2 = int(min(d["ntracks_2D"], max(n_cal, n_cal * target_s / max(t0, 1e-6))))
+    sec, segs = run(n2)
+    return {
+        "value": segs * d["G"] / sec, "unit": "integrations/s", "cores": threads,
+        "kind": "reference",
+        "sample": (f"reference OpenMP transport_sweep (-Ofast -fopenmp, {threads} threads) on the "
+                   f"first {n2} of {d['ntracks_2D']} 2D tracks of the same problem "
+                   f"({segs} 3D segment pieces x {d['G']} groups in {sec:.1f} s; calibration "
+                   f"{rate:.3g}/s on {n_cal} tracks)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tally", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--lanes-per-track", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); this build runs "
+              "the sweep on a single MI355X and has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    import simplemoc_amd as sm
+
+    t_setup = time.perf_counter()
+    inp = sm.Input(**WORKLOAD)
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=SEED)
+    log(f"problem: {inp.ntracks_2D} 2D tracks, {inp.ntracks} 3D tracks, "
+        f"{inp.n_source_regions_per_node} regions, G={inp.n_egroups} "
+        f"({time.perf_counter() - t_setup:.1f} s)")
+    opts = sm.Options(tally=args.tally, lanes_per_track=args.lanes_per_track)
+    solver = sm.Solver(prob, opts)
+    log(f"device context ready ({time.perf_counter() - t_setup:.1f} s), launch {solver.launch()}")
+
+    keff = 1.0
+
+    def step(keff):
+        segs = solver.transport_sweep()
+        t = solver.timing()
+        solver.boundary_exchange()
+        solver.renormalize_flux()
+        solver.update_sources(keff)
+        return segs, solver.compute_keff(), t
+
+    for w in range(args.warmup):
+        segs, keff, t = step(keff)
+        log(f"warmup {w}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms")
+    total_int = 0
+    att_ms, att_bytes, sweep_ms = [], [], []
+    G, T = inp.n_egroups, inp.ntracks
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        segs, keff, t = step(keff)
+        total_int += segs * G
+        att_ms.append(t["attenuate_ms"])
+        sweep_ms.append(t["sweep_ms"])
+        att_bytes.append(24.0 * segs * G + 8.0 * T * G)
+        log(f"step {k}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms "
+            f"(prep {t['prep_ms']:.1f}, trace {t['trace_ms']:.1f}, attenuate "
+            f"{t['attenuate_ms']:.1f}, finalize {t['finalize_ms']:.1f})")
+    elapsed = time.perf_counter() - t0
+    value = total_int / elapsed
+    att_avg_ms = float(np.mean(att_ms))
+    achieved = float(np.mean(att_bytes)) / (att_avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "3D segment x group integrations/s (plain transport sweep)",
+        "value": value,
+        "unit": "integrations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain sweep, one domain at full shape: 17x17 assemblies, 400 cm / 20 "
+                        "axial domains, 20 coarse x 5 fine axial intervals, 64 azimuthal x 10 "
+                        "polar, 128 groups, Poisson(40) segments",
+            "ntracks_2D": inp.ntracks_2D, "ntracks_3D": inp.ntracks,
+            "source_regions": inp.n_source_regions_per_node, "groups": G,
+            "segment_pieces_per_sweep": int(total_int // (G * args.steps)),
+            "tally": args.tally, "sweep_only_integrations_per_s": total_int / (sum(sweep_ms) * 1e-3),
+            "keff": keff,
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_attenuate", "kernel_ms": att_avg_ms},
+        "cpu_baseline": None,
+    }
+    solver.close()
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            line["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds, threads)
+        except Exception as e:  # reported, never fatal to the GPU measurement
+            log(f"cpu baseline failed: {e}")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+/*
+ * simplemoc.h -- C ABI of the MI355X (gfx950) 3D MOC transport sweep.
+ *
+ * Plain C types and pointers only.  Every function returns an int status
+ * (MOC_OK = 0) unless stated; moc_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * The per-iteration entry points mirror the reference's solver interface
+ * (SimpleMOC_header.h:225-237, called from src/main.c:61,75,81,87):
+ *
+ *   moc_transport_sweep   <-> transport_sweep(Params*, Input*)     src/solver.c:283
+ *   moc_boundary_exchange <-> fast_transfer_boundary_fluxes(...)   src/comms.c:5
+ *   moc_renormalize_flux  <-> renormalize_flux(Params, Input, CommGrid) src/solver.c:1143
+ *   moc_update_sources    <-> update_sources(Params, Input, float)  src/solver.c:1235
+ *   moc_compute_keff      <-> compute_keff(Params, Input, CommGrid) src/solver.c:1324
+ *
+ * Input handling mirrors src/init.c and src/io.c; problem generation
+ * mirrors build_tracks (src/init.c:106-159) in MOC_GEN_REFERENCE mode.
+ * The reference-signature drop-in symbols live in simplemoc_dropin.h.
+ */
+#ifndef SIMPLEMOC_H
+#define SIMPLEMOC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MOC_OK 0
+#define MOC_ERR_INVALID 1     /* bad argument or input value */
+#define MOC_ERR_IO 2          /* input file could not be read */
+#define MOC_ERR_NOMEM 3       /* host or device allocation failed */
+#define MOC_ERR_HIP 4         /* HIP runtime error */
+#define MOC_ERR_DEVICE 5      /* no usable gfx950 device */
+#define MOC_ERR_STATE 6       /* device-side consistency check failed */
+
+/* ---- user inputs: the reference Input struct (SimpleMOC_header.h:28-76) ---- */
+typedef struct moc_input {
+    int x_assemblies;            /* printed only, as in the reference */
+    int y_assemblies;
+    int cai;                     /* coarse axial intervals */
+    int fai;                     /* fine axial intervals per coarse interval */
+    int axial_exp;               /* 0 flat source, 2 quadratic source */
+    float radial_ray_sep;
+    float axial_z_sep;
+    int n_azimuthal;             /* halved by moc_derive_inputs, like the reference */
+    int n_polar_angles;
+    int n_egroups;
+    int decompose;
+    int decomp_assemblies_ax;
+    long segments_per_track;
+    float assembly_width;
+    float height;
+    float domain_height;         /* derived */
+    float precision;
+    long n_2D_source_regions_per_assembly;
+    int papi_event_set;          /* read and ignored */
+    /* derived (src/init.c:4-30) */
+    long ntracks_2D;
+    int z_stacked;
+    long ntracks;
+    long n_source_regions_per_node;
+    int derived;                 /* set by moc_derive_inputs */
+} moc_input;
+
+/* src/init.c:33-74 */
+void moc_default_input(moc_input *in);
+/* src/init.c:77-103 (the -s problem) */
+void moc_small_input(moc_input *in);
+/* src/io.c:198-270: the 18-field positional input file */
+int moc_read_input_file(const char *path, moc_input *in);
+/* src/init.c:4-30; refuses a second call on the same struct */
+int moc_derive_inputs(moc_input *in);
+
+/* ---- problem data (host, flat arrays) ---- */
+typedef struct moc_problem {
+    moc_input in;
+    long n_seg2d;
+    long n_mat;                  /* n_source_regions_per_node / 8, at least 1 */
+    float *az_weight;            /* [ntracks_2D] */
+    long *seg_offset;            /* [ntracks_2D + 1] */
+    float *seg_length;           /* [n_seg2d] */
+    float *polar_angle;          /* [n_polar_angles] */
+    float *p_weight;             /* [ntracks], t = (i*n_polar + j)*z_stacked + k */
+    float *z_height;             /* [ntracks] */
+    float *f_psi;                /* [ntracks*G] or NULL (all zero) */
+    float *b_psi;                /* [ntracks*G] or NULL (all zero) */
+    float *scatter;              /* [n_mat][G][G], scatter[m][g][g2]: g2 -> g */
+    float *xs;                   /* [n_mat][G][3] = nuSigmaF, SigmaA, chi */
+    int *material;               /* [nsr] */
+    float *vol;                  /* [nsr] */
+    float *fine_source;          /* [nsr][fai][G] */
+    float *fine_flux;            /* [nsr][fai][G] */
+    float *sigT;                 /* [nsr][G] */
+    float leakage;
+    int table_N;                 /* exponential table (src/utils.c:48-78) */
+    float table_dx;
+    float table_maxVal;
+    float *table;                /* [2*table_N] slope, intercept */
+} moc_problem;
+
+/* generator modes */
+#define MOC_GEN_SYNTHETIC 0      /* seeded counter RNG, distributions below */
+#define MOC_GEN_REFERENCE 1      /* build_tracks' libc rand() sequence after srand(seed) */
+
+/* segment-count distributions (synthetic mode) */
+#define MOC_SEGS_POISSON 0       /* Poisson(segments_per_track) */
+#define MOC_SEGS_UNIFORM 1       /* uniform integer in [seg_lo, seg_hi] */
+#define MOC_SEGS_NORMAL 2        /* N(spt, sqrt(spt)) like src/tracks.c:62-65 */
+
+typedef struct moc_gen_params {
+    int mode;                    /* MOC_GEN_* */
+    uint64_t seed;
+    int seg_dist;                /* MOC_SEGS_* */
+    long seg_lo, seg_hi;         /* MOC_SEGS_UNIFORM bounds */
+    float len_lo, len_hi;        /* 2D segment length U(len_lo, len_hi) cm */
+    float sig_lo, sig_hi;        /* sigma_t U(sig_lo, sig_hi) per region and group */
+    int alloc_psi;               /* allocate zeroed f_psi on the host */
+    int alloc_b_psi;             /* allocate zeroed b_psi on the host */
+} moc_gen_params;
+
+/* Defaults: synthetic, seed 1, Poisson, lengths U(0.01,0.5), sigma U(0.1,2.0),
+ * no host psi. */
+void moc_gen_defaults(moc_gen_params *gp);
+int moc_problem_generate(const moc_input *in, const moc_gen_params *gp, moc_problem **out);
+void moc_problem_free(moc_problem *p);
+/* polar angles pi*(j+1/2)/n_polar (src/tracks.c:159-168) and the table,
+ * for problems assembled by the caller */
+int moc_build_polar_angles(int n_polar, float *out);
+int moc_table_size(float precision, float max_val);
+int moc_build_table(float precision, float max_val, float *vals, int *N, float *dx, float *max_v);
+
+/* ---- solver options ---- */
+#define MOC_QSR_HASH 0           /* counter hash of (seed, sweep, track, segment, piece) */
+#define MOC_QSR_LIBC 1           /* rand() % nsr drawn on the host in serial trace order */
+
+#define MOC_TALLY_F64 0          /* per-sweep FP64 accumulator, rounded once */
+#define MOC_TALLY_F32 1          /* FP32 atomics straight into the flux */
+
+#define MOC_EXP_TABLE 0          /* interpolation table (reference) */
+#define MOC_EXP_INTRINSIC 1      /* -expm1f(-tau) */
+
+#define MOC_BC_REFLECTIVE 0      /* psi carries over per track, leakage 0 (reference, no MPI) */
+#define MOC_BC_VACUUM 1          /* border-face chunks tallied into leakage and zeroed */
+
+typedef struct moc_options {
+    int qsr_mode;
+    uint64_t seed;
+    int tally;
+    int exp_mode;
+    int boundary;
+    int keep_b_psi;              /* keep a b_psi array on the device */
+    int lanes_per_track;         /* 0 auto; 8, 16, 32 or 64 */
+    int device;
+} moc_options;
+
+void moc_default_options(moc_options *o);
+
+/* ---- device context ---- */
+typedef struct moc_ctx moc_ctx;
+
+int moc_create(const moc_problem *p, const moc_options *o, moc_ctx **out);
+int moc_destroy(moc_ctx *ctx);
+
+/* src/solver.c:283-552.  Synchronous.  *segments_processed may be NULL. */
+int moc_transport_sweep(moc_ctx *ctx, long *segments_processed);
+/* src/comms.c:5-196 for one domain: no-op (reflective) or vacuum tally. */
+int moc_boundary_exchange(moc_ctx *ctx);
+/* src/solver.c:1143-1230, FP64 fission-rate reduction. */
+int moc_renormalize_flux(moc_ctx *ctx);
+/* src/solver.c:1235-1320; *residual may be NULL. */
+int moc_update_sources(moc_ctx *ctx, float keff, float *residual);
+/* src/solver.c:1324-1437, FP64 reductions. */
+int moc_compute_keff(moc_ctx *ctx, float *keff);
+
+/* copy device state back into a host problem of the same shape */
+#define MOC_DL_FLUX 1u
+#define MOC_DL_SOURCE 2u
+#define MOC_DL_PSI 4u
+#define MOC_DL_Z 8u
+#define MOC_DL_LEAKAGE 16u
+#define MOC_DL_B_PSI 32u
+#define MOC_DL_ALL 63u
+int moc_download(moc_ctx *ctx, moc_problem *dst, unsigned what);
+/* copy host state (flux, source, psi, z, leakage) into the context */
+int moc_upload_state(moc_ctx *ctx, const moc_problem *src, unsigned what);
+
+/* device timings of the last call of each phase, in milliseconds */
+typedef struct moc_timing {
+    double sweep_ms;             /* all sweep kernels, first to last */
+    double prep_ms;              /* source-fit coefficients + tally accumulator */
+    double trace_ms;             /* trace count, scan, trace write */
+    double attenuate_ms;         /* attenuation kernel only */
+    double finalize_ms;
+    double boundary_ms;
+    double renorm_ms;
+    double update_ms;
+    double keff_ms;
+    long segments;               /* 3D segment pieces of the last sweep */
+    long sweeps;                 /* sweeps done */
+} moc_timing;
+int moc_get_timing(moc_ctx *ctx, moc_timing *t);
+
+/* lanes per track and groups per lane chosen for this context */
+int moc_get_launch(moc_ctx *ctx, int *lanes_per_track, int *groups_per_lane);
+
+/* Read back the trace records of one track from the last sweep:
+ * up to cap pieces of (ds, zin, fsr); returns the piece count in *n. */
+int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
+                          uint32_t *fsr, long cap, long *n);
+
+const char *moc_last_error(void);
+int moc_device_count(int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,610 @@
+/*
+ * moc_oracle.c -- CPU restatement of SimpleMOC's transport-sweep path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, smoke() and
+ * bench.py's cpu_baseline leg; never by the product.
+ *
+ * Build: plain C99, -O2 -ffp-contract=off, no -ffast-math, so float and
+ * double expressions round exactly as a -O0 x86-64 build of the reference
+ * does (SSE arithmetic, no contraction, IEEE denormals).
+ *
+ * Each function names the reference lines it follows.  Where the
+ * reference has undefined behaviour, the defined replacement is stated
+ * at that point.
+ */
+#include "moc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* src/utils.c:29-45 -- recursive halving, sequential for n <= 16.      */
+float orc_pairwise_sum(const float *v, long n)
+{
+    if (n <= 16) {
+        float acc = 0.0f;
+        for (long i = 0; i < n; i++)
+            acc += v[i];
+        return acc;
+    }
+    long h = n / 2;
+    return orc_pairwise_sum(v, h) + orc_pairwise_sum(v + h, n - h);
+}
+
+/* Pairwise sum over an element function (same tree as orc_pairwise_sum). */
+typedef float (*elem_fn)(const void *ctx, long i);
+static float pairwise_fn(elem_fn f, const void *ctx, long base, long n)
+{
+    if (n <= 16) {
+        float acc = 0.0f;
+        for (long i = 0; i < n; i++)
+            acc += f(ctx, base + i);
+        return acc;
+    }
+    long h = n / 2;
+    return pairwise_fn(f, ctx, base, h) + pairwise_fn(f, ctx, base + h, n - h);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/utils.c:54 -- number of table cells. */
+int orc_table_N(float precision, float maxVal)
+{
+    return (int)((double)maxVal * sqrt(1.0 / (8.0 * (double)precision * 0.01)));
+}
+
+/* src/utils.c:48-78 -- slope/intercept pairs for 1 - exp(-x).  The stored
+ * slope is -exp(-n dx) (the reference's sign, kept). */
+void orc_build_table(float precision, float maxVal, float *vals, float *dx_out,
+                     float *maxv_out)
+{
+    int N = orc_table_N(precision, maxVal);
+    float dx = maxVal / (float)N;
+    for (int n = 0; n < N; n++) {
+        float arg = (float)(-n) * dx;
+        float e = (float)exp((double)arg);
+        vals[2 * n] = -e;
+        vals[2 * n + 1] = 1.0f + ((float)n * dx - 1.0f) * e;
+    }
+    *dx_out = dx;
+    *maxv_out = maxVal - dx;
+}
+
+/* src/solver.c:1441-1464.  Index (int)(x/dx + 0.5f*dx) as written.  For
+ * x below about -0.028 the reference indexes before the table (undefined
+ * behaviour); the index is clamped to 0 there.  The upper clamp is never
+ * reached because x <= maxVal keeps the index at or below N-1. */
+float orc_interpolate(const float *vals, int N, float dx, float maxVal, float x)
+{
+    if (x > maxVal)
+        return 1.0f;
+    float v = x / dx + 0.5f * dx;
+    int iv;
+    if (!(v > -1.0f))          /* negative index, or NaN */
+        iv = 0;
+    else if (v >= (float)N)
+        iv = N - 1;
+    else
+        iv = (int)v;           /* truncation toward zero, as the reference */
+    return vals[2 * iv] * x + vals[2 * iv + 1];
+}
+
+/* ------------------------------------------------------------------ */
+/* Counter-based region draw (replaces rand_r at src/solver.c:477-483,
+ * which is time-seeded and order dependent).  splitmix64 finaliser. */
+static uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+
+uint32_t orc_qsr_hash(uint64_t seed, long sweep, long track, long n, long p, long nsr)
+{
+    uint64_t h = mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sweep + 1));
+    h = mix64(h ^ ((uint64_t)track * 0xD1B54A32D192ED03ULL));
+    h = mix64(h + (((uint64_t)n << 32) | (uint64_t)(uint32_t)p));
+    return (uint32_t)(((h >> 32) * (uint64_t)nsr) >> 32);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/init.c:4-30 */
+void orc_derive_shape(orc_shape *s)
+{
+    s->n_azimuthal_half = s->n_azimuthal / 2;
+    double per = (double)s->assembly_width * sqrt(2.0) / (double)s->radial_ray_sep;
+    long nt = (long)((double)s->n_azimuthal_half * per);
+    s->ntracks_2D = 2 * (nt / 2);
+    s->z_stacked = (int)(s->height / (s->axial_z_sep * (float)s->decomp_assemblies_ax));
+    s->ntracks = s->ntracks_2D * s->n_polar_angles * s->z_stacked;
+    s->domain_height = s->height / (float)s->decomp_assemblies_ax;
+    s->nsr = s->n_2D_source_regions_per_assembly * s->cai / s->decomp_assemblies_ax;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/solver.c:895-907 (dz arrives as float: the parameter type). */
+static int pos_interval(float z, float dz)
+{
+    return (int)(z / dz);
+}
+static int neg_interval(float z, float dz)
+{
+    return (int)ceil((double)(z / dz));
+}
+
+typedef struct {
+    orc_piece *v;
+    long n;
+    long cap;
+    int grow;     /* 1: realloc as needed; 0: fixed buffer, count beyond cap */
+} piece_sink;
+
+static void sink_push(piece_sink *s, const orc_piece *pc)
+{
+    if (s->n >= s->cap) {
+        if (!s->grow) {
+            s->n++;
+            return;
+        }
+        long nc = s->cap ? 2 * s->cap : 1024;
+        s->v = (orc_piece *)realloc(s->v, (size_t)nc * sizeof(orc_piece));
+        s->cap = nc;
+    }
+    if (s->v)
+        s->v[s->n] = *pc;
+    s->n++;
+}
+
+/* src/solver.c:366-527 for one (i, j) stack: polar loop body.
+ * Keeps the reference's behaviour exactly, including the live change of
+ * [begin, end) inside the k loop and the z reset on every piece of the
+ * last 2D segment. */
+static int trace_stack(const orc_problem *P, orc_opts *o, long i, int j,
+                       float *z, piece_sink *sink)
+{
+    const int zs = P->z_stacked;
+    const double node_dz = (double)(P->height / (float)P->decomp);       /* :288 */
+    const double fine_dz = node_dz / (double)(P->cai * P->fai);           /* :289 */
+    const float dzf = (float)fine_dz;
+    const int pos = (j < P->n_polar / 2);                                 /* :370-371 */
+    const float p_angle = P->polar_angle[j];
+    const double c = cos((double)p_angle);
+    const double sn = sin((double)p_angle);
+    int begin = 0, end = zs;
+    const long off = P->seg_offset[i];
+    const long nseg = P->seg_offset[i + 1] - off;
+
+    for (long n = 0; n < nseg; n++) {
+        const float s_full = (float)((double)P->seg_length[off + n] / sn);   /* :382 */
+        for (int k = begin; k < end; k++) {
+            float s = s_full;
+            const long t = ((long)i * P->n_polar + j) * zs + k;
+            int complete = 0;
+            int curr = pos ? pos_interval(z[k], dzf) : neg_interval(z[k], dzf);
+            int p = 0;
+            while (!complete) {
+                int reset = 0;
+                float ds;
+                float zn = (float)((double)z[k] + (double)s * c);            /* :417 */
+                int ni = pos ? pos_interval(zn, dzf) : neg_interval(zn, dzf);
+                if (ni == curr) {
+                    complete = 1;
+                    ds = s;
+                } else {
+                    curr += pos ? 1 : -1;
+                    zn = (float)(fine_dz * (double)(float)curr);             /* :441,445 */
+                    ds = (float)((double)(zn - z[k]) / c);                   /* :449 */
+                    s -= ds;
+                    if (s <= 0)
+                        complete = 1;
+                    if (zn <= 0 || (double)zn >= node_dz) {                  /* :460 */
+                        complete = 1;
+                        if (pos)
+                            end--;
+                        else
+                            begin++;
+                        reset = 1;
+                    }
+                }
+                long qsr;
+                if (o->qsr_mode == ORC_QSR_LIBC) {
+                    qsr = rand() % P->nsr;
+                } else if (o->qsr_mode == ORC_QSR_STREAM) {
+                    if (!o->qsr_stream || o->qsr_stream_pos >= o->qsr_stream_len)
+                        return -1;
+                    qsr = o->qsr_stream[o->qsr_stream_pos++];
+                } else {
+                    qsr = (long)orc_qsr_hash(o->seed, o->sweep_index, t, n, p, P->nsr);
+                }
+                orc_piece pc;
+                pc.track = t;
+                pc.n = (int)n;
+                pc.p = p;
+                pc.zh = z[k];
+                pc.ds = ds;
+                pc.qsr = qsr;
+                sink_push(sink, &pc);
+                p++;
+                if (n == nseg - 1 || reset)                                   /* :514-521 */
+                    z[k] = pos ? P->axial_z_sep * (float)k : P->axial_z_sep * (float)(k + 1);
+                else
+                    z[k] = zn;
+            }
+        }
+    }
+    return 0;
+}
+
+long orc_trace_stack(const orc_problem *P, orc_opts *o, long i, int j,
+                     float *z_state, orc_piece *out, long cap)
+{
+    piece_sink s;
+    s.v = out;
+    s.n = 0;
+    s.cap = out ? cap : 0;
+    s.grow = 0;
+    if (trace_stack(P, o, i, j, z_state, &s) != 0)
+        return -1;
+    return s.n;
+}
+
+/* ------------------------------------------------------------------ */
+static float expval(const orc_problem *P, const orc_opts *o, float tau)
+{
+    if (o->exp_mode == ORC_EXP_INTRINSIC)
+        return -expm1f(-tau);
+    return orc_interpolate(P->table, P->table_N, P->table_dx, P->table_maxVal, tau);
+}
+
+/* src/solver.c:14-280 (axial_exp == 2) and src/solver.c:1040-1138
+ * (axial_exp == 0).  For fai < 3 the reference reads source rows outside
+ * the region (quirk: rows fine_id+-2); here the row indices are clamped
+ * into [0, fai-1] (defined fallback, a no-op for fai >= 3). */
+void orc_attenuate(const orc_problem *P, const orc_opts *o, long t, float zh,
+                   float ds, long qsr, float mu, float azw, float *psi,
+                   float *flux_f32, double *acc_f64)
+{
+    const int G = P->G;
+    const int fai = P->fai;
+    const float dz = P->height / (float)(P->fai * P->decomp * P->cai);    /* :38 */
+    const int cell = (int)(zh / dz);
+    float zin = zh - dz * ((float)cell + 0.5f);                           /* :41-42 */
+    const int fine_id = cell % fai;                                       /* :45 */
+    const long fsr = qsr * fai + fine_id;
+    const float *src = P->fine_source + qsr * fai * G;
+    const float *sig = P->sigT + qsr * G;
+
+    if (P->axial_exp == 0) {
+        const float weight = P->p_weight[t] * azw * mu;                   /* :1064 */
+        for (int g = 0; g < G; g++) {
+            float sigv = sig[g];
+            float tau = sigv * ds;
+            float F = expval(P, o, tau);
+            float q = src[fine_id * G + g] / sigv;                        /* :1107 */
+            float dpsi = (psi[g] - q) * F;
+            float tally = weight * dpsi;
+            psi[g] -= dpsi;
+            if (acc_f64)
+                acc_f64[fsr * G + g] += (double)tally;
+            else if (flux_f32)
+                flux_f32[fsr * G + g] += tally;
+        }
+        return;
+    }
+
+    const float weight = P->p_weight[t] * azw;                            /* :49 */
+    const float mu2 = mu * mu;
+    int r1, r2, r3;
+    if (fine_id == 0) {
+        zin -= dz;
+        r1 = 0; r2 = 1; r3 = 2;
+    } else if (fine_id == fai - 1) {
+        zin += dz;
+        r1 = fine_id - 2; r2 = fine_id - 1; r3 = fine_id;
+    } else {
+        r1 = fine_id - 1; r2 = fine_id; r3 = fine_id + 1;
+    }
+    if (r1 < 0) r1 = 0;
+    if (r2 < 0) r2 = 0;
+    if (r1 > fai - 1) r1 = fai - 1;
+    if (r2 > fai - 1) r2 = fai - 1;
+    if (r3 > fai - 1) r3 = fai - 1;
+
+    const float two_dz = 2.f * dz;
+    const float two_dz2 = 2.f * dz * dz;
+    for (int g = 0; g < G; g++) {
+        float y1 = src[r1 * G + g];
+        float y2 = src[r2 * G + g];
+        float y3 = src[r3 * G + g];
+        float c0 = y2;                                                    /* :74 */
+        float c1 = (y1 - y3) / two_dz;
+        float c2 = (y1 - 2.f * y2 + y3) / two_dz2;
+        float q0 = c0 + c1 * zin + c2 * zin * zin;                        /* :79 */
+        float q1 = c1 + 2.f * c2 * zin;
+        float q2 = c2;
+
+        float sigv = sig[g];                                              /* :149-153 */
+        float tau = sigv * ds;
+        float sig2 = sigv * sigv;
+        float F = expval(P, o, tau);
+        float reuse = tau * (tau - 2.f) + 2.f * F / (sigv * sig2);       /* :175-176 */
+        float ps = psi[g];
+        float fi = (q0 * tau + (sigv * ps - q0) * F) / sig2 + q1 * mu * reuse
+                 + q2 * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F)
+                 / (3.f * sig2 * sig2);                                   /* :195-198 */
+        float tally = weight * fi;                                        /* :209 */
+        if (acc_f64)
+            acc_f64[fsr * G + g] += (double)tally;
+        else if (flux_f32)
+            flux_f32[fsr * G + g] += tally;
+        float t1 = q0 * F / sigv;                                         /* :238 */
+        float t2 = q1 * mu * (tau - F) / sig2;                            /* :248 */
+        float t3 = q2 * mu2 * reuse;                                      /* :258 */
+        float t4 = ps * (1.f - F);                                        /* :268 */
+        psi[g] = t1 + t2 + t3 + t4;                                       /* :278 */
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/solver.c:283-552 in serial order.  Each stack is traced first and
+ * its pieces attenuated afterwards in the recorded order; attenuation does
+ * not feed back into the trace, and the region draws happen in the same
+ * order, so this equals the reference's interleaved loop. */
+long orc_transport_sweep(orc_problem *P, orc_opts *o)
+{
+    if (P->axial_exp != 0 && P->axial_exp != 2)
+        return -1;                                                        /* :506-511 */
+    const int G = P->G;
+    const long nval = P->nsr * P->fai * G;
+    double *acc = NULL;
+    if (o->tally == ORC_TALLY_F64) {
+        acc = (double *)malloc((size_t)nval * sizeof(double));
+        for (long e = 0; e < nval; e++)
+            acc[e] = (double)P->fine_flux[e];
+    }
+    piece_sink sink;
+    memset(&sink, 0, sizeof(sink));
+    sink.grow = 1;
+    long total = 0;
+    for (long i = 0; i < P->ntracks_2D; i++) {
+        const float azw = P->az_weight[i];
+        for (int j = 0; j < P->n_polar; j++) {
+            const float mu = (float)cos((double)P->polar_angle[j]);       /* :373 */
+            float *z = P->z_height + ((long)i * P->n_polar + j) * P->z_stacked;
+            sink.n = 0;
+            if (trace_stack(P, o, i, j, z, &sink) != 0) {
+                free(sink.v);
+                free(acc);
+                return -1;
+            }
+            for (long q = 0; q < sink.n; q++) {
+                const orc_piece *pc = &sink.v[q];
+                orc_attenuate(P, o, pc->track, pc->zh, pc->ds, pc->qsr, mu, azw,
+                              P->f_psi + pc->track * G, P->fine_flux, acc);
+            }
+            total += sink.n;
+        }
+    }
+    if (acc) {
+        for (long e = 0; e < nval; e++)
+            P->fine_flux[e] = (float)acc[e];
+        free(acc);
+    }
+    free(sink.v);
+    o->sweep_index++;
+    return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/comms.c:5-196 on a domain whose six neighbours are all absent: every
+ * face chunk is a border chunk, so its pairwise sum goes to leakage and
+ * the received buffer (zeros) is copied back over it.  The flat flux
+ * buffer is the interleaved [track][f|b][G] block (src/tracks.c:111-138);
+ * b values read as 0 when b_psi is NULL. */
+static float interleaved_at(const orc_problem *P, long e)
+{
+    const long G = P->G;
+    long t = e / (2 * G);
+    long r = e % (2 * G);
+    if (r < G)
+        return P->f_psi[t * G + r];
+    return P->b_psi ? P->b_psi[t * G + (r - G)] : 0.0f;
+}
+static float interleaved_fn(const void *ctx, long e)
+{
+    return interleaved_at((const orc_problem *)ctx, e);
+}
+
+void orc_boundary_vacuum(orc_problem *P, const orc_opts *o)
+{
+    const long tpm = 10000;                                              /* :10 */
+    const float h = P->domain_height;
+    const float x = P->assembly_width;
+    const long T = P->ntracks;
+    long ax = (long)((float)T * x / (2 * x + 4 * h));                    /* :16 */
+    long rad = (long)((float)T * h / (2 * x + 4 * h));                   /* :17 */
+    long rem = T - 2 * ax - 4 * rad;
+    long add_rad = (long)((float)rem * (4 * h / (2 * x + 4 * h)));       /* :23 */
+    add_rad = 4 * (add_rad / 4);
+    rad += add_rad / 4;
+    long add_ax = rem - add_rad;
+    ax += add_ax / 2;
+    long nmsg[6] = {rad / tpm, rad / tpm, rad / tpm, rad / tpm, ax / tpm, ax / tpm};
+    long maxm = 0;
+    for (int f = 0; f < 6; f++)
+        if (nmsg[f] > maxm)
+            maxm = nmsg[f];
+    const long chunk = (long)P->G * tpm;
+    long idx = 0;
+    double sum_d = 0.0;
+    for (long m = 0; m < maxm; m++) {
+        for (int f = 0; f < 6; f++) {
+            if (m >= nmsg[f])
+                continue;
+            if (o->reduce == ORC_REDUCE_F32) {
+                *P->leakage += pairwise_fn(interleaved_fn, P, idx, chunk);
+            } else {
+                for (long e = idx; e < idx + chunk; e++)
+                    sum_d += (double)interleaved_at(P, e);
+            }
+            idx += chunk;
+        }
+    }
+    if (o->reduce == ORC_REDUCE_F64)
+        *P->leakage = (float)((double)*P->leakage + sum_d);
+    /* zero the exchanged range: idx / (2G) whole tracks */
+    long nt = idx / (2 * (long)P->G);
+    memset(P->f_psi, 0, (size_t)nt * P->G * sizeof(float));
+    if (P->b_psi)
+        memset(P->b_psi, 0, (size_t)nt * P->G * sizeof(float));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/solver.c:1143-1230, serial build (OPENMP=no) semantics. */
+void orc_renormalize_flux(orc_problem *P, const orc_opts *o)
+{
+    const int G = P->G, fai = P->fai;
+    const long nsr = P->nsr;
+    float nf;
+    if (o->reduce == ORC_REDUCE_F32) {
+        float *fr = (float *)malloc((size_t)nsr * sizeof(float));
+        float *ffr = (float *)malloc((size_t)fai * sizeof(float));
+        float *gfr = (float *)malloc((size_t)G * sizeof(float));
+        for (long i = 0; i < nsr; i++) {
+            const float *xs = P->xs + (long)P->material[i] * G * 3;
+            for (int j = 0; j < fai; j++) {
+                const float *phi = P->fine_flux + (i * fai + j) * G;
+                for (int g = 0; g < G; g++)
+                    gfr[g] = phi[g] * P->vol[i] * xs[g * 3 + 0];          /* :1167 */
+                ffr[j] = orc_pairwise_sum(gfr, G);
+            }
+            fr[i] = orc_pairwise_sum(ffr, fai);
+        }
+        float total = orc_pairwise_sum(fr, nsr);
+        free(fr);
+        free(ffr);
+        free(gfr);
+        nf = (float)(1.0 / (double)total);                               /* :1203 */
+    } else {
+        double total = 0.0;
+        for (long i = 0; i < nsr; i++) {
+            const float *xs = P->xs + (long)P->material[i] * G * 3;
+            for (int j = 0; j < fai; j++) {
+                const float *phi = P->fine_flux + (i * fai + j) * G;
+                for (int g = 0; g < G; g++)
+                    total += (double)(phi[g] * P->vol[i] * xs[g * 3 + 0]);
+            }
+        }
+        nf = (float)(1.0 / total);
+    }
+    for (long i = 0; i < nsr; i++) {
+        float adjust = (float)((double)(nf * 4.0f) * M_PI * (double)fai
+                               / (double)P->vol[i]);                      /* :1210 */
+        float *phi = P->fine_flux + i * fai * G;
+        for (long e = 0; e < (long)fai * G; e++)
+            phi[e] *= adjust;
+    }
+    const long nv = P->ntracks * G;
+    for (long e = 0; e < nv; e++)
+        P->f_psi[e] *= nf;                                                /* :1224 */
+    if (P->b_psi)
+        for (long e = 0; e < nv; e++)
+            P->b_psi[e] *= nf;                                            /* :1225 */
+}
+
+/* ------------------------------------------------------------------ */
+/* src/solver.c:1235-1320 */
+float orc_update_sources(orc_problem *P, float keff)
+{
+    const int G = P->G, fai = P->fai;
+    const long nsr = P->nsr;
+    const float inverse_k = (float)(1.0 / (double)keff);                 /* :1241 */
+    float *group_res = (float *)malloc((size_t)G * sizeof(float));
+    float *fine_res = (float *)malloc((size_t)fai * sizeof(float));
+    float *residuals = (float *)malloc((size_t)nsr * sizeof(float));
+    float *fis_r = (float *)malloc((size_t)G * sizeof(float));
+    float *sc_r = (float *)malloc((size_t)G * sizeof(float));
+    for (long i = 0; i < nsr; i++) {
+        const long m = P->material[i];
+        const float *xs = P->xs + m * G * 3;
+        const float *S = P->scatter + m * (long)G * G;
+        for (int j = 0; j < fai; j++) {
+            const float *phi = P->fine_flux + (i * fai + j) * G;
+            float *src = P->fine_source + (i * fai + j) * G;
+            for (int g = 0; g < G; g++)
+                fis_r[g] = phi[g] * xs[g * 3 + 0];                        /* :1267 */
+            float fission = orc_pairwise_sum(fis_r, G);
+            fission *= inverse_k;
+            for (int g = 0; g < G; g++) {
+                for (int g2 = 0; g2 < G; g2++)
+                    sc_r[g2] = S[(long)g * G + g2] * phi[g2];             /* :1279 */
+                float scatter = orc_pairwise_sum(sc_r, G);
+                float chi = xs[g * 3 + 2];
+                float newSrc = (float)((double)(fission * chi + scatter) / (4.0 * M_PI));
+                float oldSrc = src[g];
+                group_res[g] = (newSrc - oldSrc) * (newSrc - oldSrc) / (oldSrc * oldSrc);
+                src[g] = newSrc;
+            }
+            fine_res[j] = orc_pairwise_sum(group_res, G);
+        }
+        residuals[i] = orc_pairwise_sum(fine_res, fai);
+    }
+    float residual = orc_pairwise_sum(residuals, nsr);
+    free(group_res);
+    free(fine_res);
+    free(residuals);
+    free(fis_r);
+    free(sc_r);
+    return residual;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/solver.c:1324-1437 (non-MPI branch at :1425). */
+float orc_compute_keff(const orc_problem *P, const orc_opts *o)
+{
+    const int G = P->G, fai = P->fai;
+    const long nsr = P->nsr;
+    if (o->reduce == ORC_REDUCE_F32) {
+        float *gr = (float *)malloc((size_t)G * sizeof(float));
+        float *fr = (float *)malloc((size_t)fai * sizeof(float));
+        float *qr = (float *)malloc((size_t)nsr * sizeof(float));
+        float node[2];
+        for (int which = 0; which < 2; which++) {
+            const int col = which == 0 ? 1 : 0;  /* absorption first, then fission */
+            for (long i = 0; i < nsr; i++) {
+                const float *xs = P->xs + (long)P->material[i] * G * 3;
+                for (int j = 0; j < fai; j++) {
+                    const float *phi = P->fine_flux + (i * fai + j) * G;
+                    for (int g = 0; g < G; g++)
+                        gr[g] = xs[g * 3 + col] * phi[g];                 /* :1347,1373 */
+                    fr[j] = orc_pairwise_sum(gr, G);
+                }
+                qr[i] = orc_pairwise_sum(fr, fai);
+            }
+            node[which] = orc_pairwise_sum(qr, nsr);
+        }
+        free(gr);
+        free(fr);
+        free(qr);
+        return node[1] / (node[0] + *P->leakage);                         /* :1425 */
+    }
+    double abs_d = 0.0, fis_d = 0.0;
+    for (long i = 0; i < nsr; i++) {
+        const float *xs = P->xs + (long)P->material[i] * G * 3;
+        for (int j = 0; j < fai; j++) {
+            const float *phi = P->fine_flux + (i * fai + j) * G;
+            for (int g = 0; g < G; g++) {
+                abs_d += (double)(xs[g * 3 + 1] * phi[g]);
+                fis_d += (double)(xs[g * 3 + 0] * phi[g]);
+            }
+        }
+    }
+    return (float)(fis_d / (abs_d + (double)*P->leakage));
+}

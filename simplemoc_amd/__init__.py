@@ -1,0 +1,453 @@
+"""simplemoc_amd -- 3D MOC transport sweep on AMD Instinct MI355X (gfx950).
+
+Python mirror of the reference's solver interface over the C ABI in
+include/simplemoc.h (library: simplemoc_amd/_lib/libsimplemoc_hip.so).
+
+    inp = Input.from_file("default.in")           # src/io.c:198-270
+    inp.derive()                                  # src/init.c:4-30
+    prob = Problem.generate(inp, seed=1)          # build_tracks, src/init.c:106
+    solver = Solver(prob)                         # device context on cuda:0
+    keff = 1.0
+    for it in range(n):                           # src/main.c:57-92
+        solver.transport_sweep()
+        solver.boundary_exchange()
+        solver.renormalize_flux()
+        res = solver.update_sources(keff)
+        keff = solver.compute_keff()
+
+Every call runs HIP kernels; there is no CPU fallback.  Loading the package
+without the built library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Input", "Problem", "Options", "Solver", "MOCError", "lib", "LIB_PATH"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsimplemoc_hip.so")
+DRIVER_PATH = os.path.join(_HERE, "_lib", "simplemoc")
+
+# status codes (include/simplemoc.h)
+MOC_OK = 0
+GEN_SYNTHETIC, GEN_REFERENCE = 0, 1
+SEGS_POISSON, SEGS_UNIFORM, SEGS_NORMAL = 0, 1, 2
+QSR_HASH, QSR_LIBC = 0, 1
+TALLY_F64, TALLY_F32 = 0, 1
+EXP_TABLE, EXP_INTRINSIC = 0, 1
+BC_REFLECTIVE, BC_VACUUM = 0, 1
+DL_FLUX, DL_SOURCE, DL_PSI, DL_Z, DL_LEAKAGE, DL_B_PSI, DL_ALL = 1, 2, 4, 8, 16, 32, 63
+
+
+class MOCError(RuntimeError):
+    pass
+
+
+class _Input(C.Structure):
+    _fields_ = [
+        ("x_assemblies", C.c_int), ("y_assemblies", C.c_int), ("cai", C.c_int),
+        ("fai", C.c_int), ("axial_exp", C.c_int), ("radial_ray_sep", C.c_float),
+        ("axial_z_sep", C.c_float), ("n_azimuthal", C.c_int), ("n_polar_angles", C.c_int),
+        ("n_egroups", C.c_int), ("decompose", C.c_int), ("decomp_assemblies_ax", C.c_int),
+        ("segments_per_track", C.c_long), ("assembly_width", C.c_float), ("height", C.c_float),
+        ("domain_height", C.c_float), ("precision", C.c_float),
+        ("n_2D_source_regions_per_assembly", C.c_long), ("papi_event_set", C.c_int),
+        ("ntracks_2D", C.c_long), ("z_stacked", C.c_int), ("ntracks", C.c_long),
+        ("n_source_regions_per_node", C.c_long), ("derived", C.c_int),
+    ]
+
+
+_fp = C.POINTER(C.c_float)
+
+
+class _Problem(C.Structure):
+    _fields_ = [
+        ("inp", _Input), ("n_seg2d", C.c_long), ("n_mat", C.c_long),
+        ("az_weight", _fp), ("seg_offset", C.POINTER(C.c_long)), ("seg_length", _fp),
+        ("polar_angle", _fp), ("p_weight", _fp), ("z_height", _fp), ("f_psi", _fp),
+        ("b_psi", _fp), ("scatter", _fp), ("xs", _fp), ("material", C.POINTER(C.c_int)),
+        ("vol", _fp), ("fine_source", _fp), ("fine_flux", _fp), ("sigT", _fp),
+        ("leakage", C.c_float), ("table_N", C.c_int), ("table_dx", C.c_float),
+        ("table_maxVal", C.c_float), ("table", _fp),
+    ]
+
+
+class _GenParams(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("seed", C.c_uint64), ("seg_dist", C.c_int), ("seg_lo", C.c_long),
+        ("seg_hi", C.c_long), ("len_lo", C.c_float), ("len_hi", C.c_float),
+        ("sig_lo", C.c_float), ("sig_hi", C.c_float), ("alloc_psi", C.c_int),
+        ("alloc_b_psi", C.c_int),
+    ]
+
+
+class _Options(C.Structure):
+    _fields_ = [
+        ("qsr_mode", C.c_int), ("seed", C.c_uint64), ("tally", C.c_int), ("exp_mode", C.c_int),
+        ("boundary", C.c_int), ("keep_b_psi", C.c_int), ("lanes_per_track", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+class _Timing(C.Structure):
+    _fields_ = [
+        ("sweep_ms", C.c_double), ("prep_ms", C.c_double), ("trace_ms", C.c_double),
+        ("attenuate_ms", C.c_double), ("finalize_ms", C.c_double), ("boundary_ms", C.c_double),
+        ("renorm_ms", C.c_double), ("update_ms", C.c_double), ("keff_ms", C.c_double),
+        ("segments", C.c_long), ("sweeps", C.c_long),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MOCError(f"native library missing: {LIB_PATH}; build it with "
+                           "`python -m simplemoc_amd.build` (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        P, I, G, O = C.POINTER(_Problem), C.POINTER(_Input), C.POINTER(_GenParams), C.POINTER(_Options)
+        sig = {
+            "moc_default_input": ([I], None), "moc_small_input": ([I], None),
+            "moc_read_input_file": ([C.c_char_p, I], C.c_int),
+            "moc_derive_inputs": ([I], C.c_int),
+            "moc_gen_defaults": ([G], None),
+            "moc_problem_generate": ([I, G, C.POINTER(P)], C.c_int),
+            "moc_problem_free": ([P], None),
+            "moc_build_polar_angles": ([C.c_int, _fp], C.c_int),
+            "moc_table_size": ([C.c_float, C.c_float], C.c_int),
+            "moc_build_table": ([C.c_float, C.c_float, _fp, C.POINTER(C.c_int), _fp, _fp], C.c_int),
+            "moc_default_options": ([O], None),
+            "moc_create": ([P, O, C.POINTER(C.c_void_p)], C.c_int),
+            "moc_destroy": ([C.c_void_p], C.c_int),
+            "moc_transport_sweep": ([C.c_void_p, C.POINTER(C.c_long)], C.c_int),
+            "moc_boundary_exchange": ([C.c_void_p], C.c_int),
+            "moc_renormalize_flux": ([C.c_void_p], C.c_int),
+            "moc_update_sources": ([C.c_void_p, C.c_float, _fp], C.c_int),
+            "moc_compute_keff": ([C.c_void_p, _fp], C.c_int),
+            "moc_download": ([C.c_void_p, P, C.c_uint], C.c_int),
+            "moc_upload_state": ([C.c_void_p, P, C.c_uint], C.c_int),
+            "moc_get_timing": ([C.c_void_p, C.POINTER(_Timing)], C.c_int),
+            "moc_get_launch": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+            "moc_get_track_records": ([C.c_void_p, C.c_long, _fp, _fp, C.POINTER(C.c_uint32),
+                                       C.c_long, C.POINTER(C.c_long)], C.c_int),
+            "moc_last_error": ([], C.c_char_p),
+            "moc_device_count": ([C.POINTER(C.c_int)], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != MOC_OK:
+        msg = lib().moc_last_error().decode(errors="replace")
+        raise MOCError(f"simplemoc error {rc}: {msg}")
+
+
+# --------------------------------------------------------------------------
+class Input:
+    """The reference's Input (src/SimpleMOC_header.h:28-76)."""
+
+    def __init__(self, **fields):
+        self._s = _Input()
+        lib().moc_default_input(C.byref(self._s))
+        for k, v in fields.items():
+            setattr(self, k, v)
+
+    @classmethod
+    def default(cls):
+        return cls()
+
+    @classmethod
+    def small(cls):
+        s = cls()
+        lib().moc_small_input(C.byref(s._s))
+        return s
+
+    @classmethod
+    def from_file(cls, path):
+        s = cls()
+        _check(lib().moc_read_input_file(str(path).encode(), C.byref(s._s)))
+        return s
+
+    def derive(self):
+        _check(lib().moc_derive_inputs(C.byref(self._s)))
+        return self
+
+    def __getattr__(self, k):
+        if k.startswith("_"):
+            raise AttributeError(k)
+        return getattr(self._s, k)
+
+    def __setattr__(self, k, v):
+        if k == "_s":
+            object.__setattr__(self, k, v)
+        else:
+            if k not in dict(_Input._fields_):
+                raise AttributeError(f"Input has no field {k}")
+            setattr(self._s, k, v)
+
+    def as_dict(self):
+        return {k: getattr(self._s, k) for k, _ in _Input._fields_}
+
+
+class Problem:
+    """Flat host arrays of one problem (include/simplemoc.h moc_problem)."""
+
+    _ARR = ("az_weight", "seg_offset", "seg_length", "polar_angle", "p_weight", "z_height",
+            "f_psi", "b_psi", "scatter", "xs", "material", "vol", "fine_source", "fine_flux",
+            "sigT", "table")
+
+    def __init__(self, ptr, owned, keep=None):
+        self._p = ptr
+        self._owned = owned
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._p:
+            lib().moc_problem_free(self._p)
+            self._p = None
+
+    @classmethod
+    def generate(cls, inp: Input, seed=1, mode="synthetic", seg_dist="poisson", seg_lo=5,
+                 seg_hi=500, len_lo=0.01, len_hi=0.5, sig_lo=0.1, sig_hi=2.0, alloc_psi=False,
+                 alloc_b_psi=False):
+        if not inp.derived:
+            inp.derive()
+        gp = _GenParams()
+        lib().moc_gen_defaults(C.byref(gp))
+        gp.mode = {"synthetic": GEN_SYNTHETIC, "reference": GEN_REFERENCE}[mode]
+        gp.seed = seed
+        gp.seg_dist = {"poisson": SEGS_POISSON, "uniform": SEGS_UNIFORM, "normal": SEGS_NORMAL}[seg_dist]
+        gp.seg_lo, gp.seg_hi = seg_lo, seg_hi
+        gp.len_lo, gp.len_hi, gp.sig_lo, gp.sig_hi = len_lo, len_hi, sig_lo, sig_hi
+        gp.alloc_psi, gp.alloc_b_psi = int(alloc_psi), int(alloc_b_psi)
+        out = C.POINTER(_Problem)()
+        _check(lib().moc_problem_generate(C.byref(inp._s), C.byref(gp), C.byref(out)))
+        return cls(out, True)
+
+    @classmethod
+    def from_arrays(cls, inp: Input, arrays: dict, leakage=0.0):
+        """Wrap caller-owned numpy arrays (kept alive by the returned object)."""
+        if not inp.derived:
+            raise MOCError("inputs must be derived first")
+        keep = {}
+        p = _Problem()
+        p.inp = inp._s
+        conv = {"seg_offset": np.int64, "material": np.int32}
+        for k in cls._ARR:
+            a = arrays.get(k)
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=conv.get(k, np.float32))
+            keep[k] = a
+            ct = C.c_long if k == "seg_offset" else (C.c_int if k == "material" else C.c_float)
+            setattr(p, k, a.ctypes.data_as(C.POINTER(ct)))
+        p.n_seg2d = int(keep["seg_offset"][-1])
+        p.n_mat = int(keep["xs"].size // (inp.n_egroups * 3))
+        p.leakage = leakage
+        t = keep["table"]
+        p.table_N = t.size // 2
+        N = C.c_int()
+        dx = C.c_float()
+        mv = C.c_float()
+        _check(lib().moc_build_table(inp.precision, 10.0, None, C.byref(N), C.byref(dx), C.byref(mv)))
+        p.table_dx, p.table_maxVal = dx.value, mv.value
+        keep["_struct"] = p
+        return cls(C.pointer(p), False, keep)
+
+    # -- views
+    @property
+    def inp(self):
+        return self._p.contents.inp
+
+    def _shape(self, k):
+        I = self._p.contents.inp
+        P = self._p.contents
+        T, G = I.ntracks, I.n_egroups
+        nsr, fai = I.n_source_regions_per_node, I.fai
+        return {
+            "az_weight": I.ntracks_2D, "seg_offset": I.ntracks_2D + 1, "seg_length": P.n_seg2d,
+            "polar_angle": I.n_polar_angles, "p_weight": T, "z_height": T, "f_psi": T * G,
+            "b_psi": T * G, "scatter": P.n_mat * G * G, "xs": P.n_mat * G * 3, "material": nsr,
+            "vol": nsr, "fine_source": nsr * fai * G, "fine_flux": nsr * fai * G,
+            "sigT": nsr * G, "table": 2 * P.table_N,
+        }[k]
+
+    def array(self, k):
+        ptr = getattr(self._p.contents, k)
+        if not ptr:
+            return None
+        dt = {"seg_offset": np.int64, "material": np.int32}.get(k, np.float32)
+        n = self._shape(k)
+        buf = C.cast(ptr, C.POINTER(C.c_byte * (n * np.dtype(dt).itemsize))).contents
+        buf._owner = self   # the view keeps the problem (and its memory) alive
+        return np.frombuffer(buf, dtype=dt, count=n)
+
+    @property
+    def arrays(self):
+        return {k: self.array(k) for k in self._ARR}
+
+    @property
+    def dims(self):
+        """Shape fields in the form the oracle takes."""
+        I = self._p.contents.inp
+        P = self._p.contents
+        return dict(ntracks_2D=I.ntracks_2D, n_polar=I.n_polar_angles, z_stacked=I.z_stacked,
+                    G=I.n_egroups, nsr=I.n_source_regions_per_node, fai=I.fai, cai=I.cai,
+                    decomp=I.decomp_assemblies_ax, axial_exp=I.axial_exp, height=I.height,
+                    axial_z_sep=I.axial_z_sep, assembly_width=I.assembly_width,
+                    domain_height=I.domain_height, ntracks=I.ntracks, n_mat=P.n_mat,
+                    table_N=P.table_N, table_dx=P.table_dx, table_maxVal=P.table_maxVal,
+                    leakage=P.leakage)
+
+    @property
+    def leakage(self):
+        return self._p.contents.leakage
+
+
+class Options:
+    def __init__(self, qsr="hash", seed=1, tally="f64", exp="table", boundary="reflective",
+                 keep_b_psi=False, lanes_per_track=0, device=0):
+        self.qsr, self.seed, self.tally, self.exp = qsr, seed, tally, exp
+        self.boundary, self.keep_b_psi = boundary, keep_b_psi
+        self.lanes_per_track, self.device = lanes_per_track, device
+
+    def _struct(self):
+        o = _Options()
+        lib().moc_default_options(C.byref(o))
+        o.qsr_mode = {"hash": QSR_HASH, "libc": QSR_LIBC}[self.qsr]
+        o.seed = self.seed
+        o.tally = {"f64": TALLY_F64, "f32": TALLY_F32}[self.tally]
+        o.exp_mode = {"table": EXP_TABLE, "intrinsic": EXP_INTRINSIC}[self.exp]
+        o.boundary = {"reflective": BC_REFLECTIVE, "vacuum": BC_VACUUM}[self.boundary]
+        o.keep_b_psi = int(self.keep_b_psi)
+        o.lanes_per_track = self.lanes_per_track
+        o.device = self.device
+        return o
+
+
+class Solver:
+    """Device context: the reference's per-iteration solver calls on the GPU."""
+
+    def __init__(self, problem: Problem, options: Options | None = None):
+        self.problem = problem
+        self.options = options or Options()
+        h = C.c_void_p()
+        _check(lib().moc_create(problem._p, C.byref(self.options._struct()), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().moc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # src/solver.c:283
+    def transport_sweep(self) -> int:
+        n = C.c_long()
+        _check(lib().moc_transport_sweep(self._h, C.byref(n)))
+        return n.value
+
+    # src/comms.c:5
+    def boundary_exchange(self):
+        _check(lib().moc_boundary_exchange(self._h))
+
+    # src/solver.c:1143
+    def renormalize_flux(self):
+        _check(lib().moc_renormalize_flux(self._h))
+
+    # src/solver.c:1235
+    def update_sources(self, keff: float) -> float:
+        r = C.c_float()
+        _check(lib().moc_update_sources(self._h, keff, C.byref(r)))
+        return r.value
+
+    # src/solver.c:1324
+    def compute_keff(self) -> float:
+        k = C.c_float()
+        _check(lib().moc_compute_keff(self._h, C.byref(k)))
+        return k.value
+
+    def iterate(self, n_iters, keff=1.0):
+        """src/main.c:57-92 call order; returns [(segments, keff, residual)]."""
+        out = []
+        for _ in range(n_iters):
+            segs = self.transport_sweep()
+            self.boundary_exchange()
+            self.renormalize_flux()
+            res = self.update_sources(keff)
+            keff = self.compute_keff()
+            out.append((segs, keff, res))
+        return out
+
+    def download(self, what=DL_ALL, into: Problem | None = None):
+        """Copy device state into `into` (default: the problem this solver was made from)."""
+        p = into or self.problem
+        _check(lib().moc_download(self._h, p._p, what))
+        return p
+
+    def state(self, psi: bool = True):
+        """Fresh numpy copies of flux, source, (psi,) z and leakage."""
+        I = self.problem._p.contents.inp
+        T, G = I.ntracks, I.n_egroups
+        nv = I.n_source_regions_per_node * I.fai * G
+        arrs = dict(fine_flux=np.zeros(nv, np.float32), fine_source=np.zeros(nv, np.float32),
+                    z_height=np.zeros(T, np.float32))
+        what = DL_FLUX | DL_SOURCE | DL_Z | DL_LEAKAGE
+        if psi:
+            arrs["f_psi"] = np.zeros(T * G, np.float32)
+            what |= DL_PSI
+        p = _Problem()
+        p.inp = I
+        for k, a in arrs.items():
+            setattr(p, k, a.ctypes.data_as(_fp))
+        _check(lib().moc_download(self._h, C.pointer(p), what))
+        arrs["leakage"] = p.leakage
+        return arrs
+
+    def timing(self) -> dict:
+        t = _Timing()
+        _check(lib().moc_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _Timing._fields_}
+
+    def launch(self):
+        lpt, gpl = C.c_int(), C.c_int()
+        _check(lib().moc_get_launch(self._h, C.byref(lpt), C.byref(gpl)))
+        return lpt.value, gpl.value
+
+    def track_records(self, track: int, cap: int = 1 << 16):
+        ds = np.zeros(cap, np.float32)
+        zin = np.zeros(cap, np.float32)
+        fsr = np.zeros(cap, np.uint32)
+        n = C.c_long()
+        _check(lib().moc_get_track_records(self._h, track, ds.ctypes.data_as(_fp),
+                                           zin.ctypes.data_as(_fp),
+                                           fsr.ctypes.data_as(C.POINTER(C.c_uint32)), cap,
+                                           C.byref(n)))
+        m = min(n.value, cap)
+        return ds[:m].copy(), zin[:m].copy(), fsr[:m].copy()
+
+
+def device_count() -> int:
+    n = C.c_int()
+    lib().moc_device_count(C.byref(n))
+    return n.value

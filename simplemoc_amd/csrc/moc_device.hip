@@ -1,0 +1,1426 @@
+// moc_device.hip -- gfx950 kernels and the device context behind the C ABI.
+//
+// Hot path (one transport sweep, src/solver.c:283-552):
+//   k_prep        per (FSR, group): quadratic-fit coefficients {c0,c1,c2}
+//                 and sigma_t packed in one float4 (src/solver.c:55-154),
+//                 FP64 tally accumulator seeded with the old flux
+//   k_trace<0>    per z-stack: axial ray trace, piece count per track
+//   scan          piece offsets (rocPRIM exclusive scan, int64)
+//   k_trace<1>    per z-stack: same trace, writes piece records
+//                 {ds, zin, fsr} and the final z heights
+//   k_attenuate   per 3D track: groups across lanes, psi in registers,
+//                 exp table in LDS, one atomic per (piece, group)
+//   k_finalize    flux = (float)accumulator
+// Reductions (src/solver.c:1143-1437): FP64 fission/absorption sums,
+// exact FP32 pairwise trees for update_sources.
+//
+// Arithmetic that decides control flow or table cells (trace, table index)
+// is bit-identical to the reference's: IEEE division and conversions in
+// the same float/double types, no contraction (-ffp-contract=off).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include "moc_internal.h"
+#include "moc_kernels.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+using namespace moc;
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return moc_set_error(MOC_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                                      \
+    } while (0)
+
+// ---------------------------------------------------------------------
+// hashing of region draws (same definition as the oracle's qsr hash)
+__host__ __device__ static inline uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+static inline uint64_t sweep_key(uint64_t seed, long sweep)
+{
+    return mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sweep + 1));
+}
+__device__ static inline uint64_t track_key(uint64_t skey, long long t)
+{
+    return mix64(skey ^ ((uint64_t)t * 0xD1B54A32D192ED03ULL));
+}
+__device__ static inline uint32_t piece_region(uint64_t tkey, long long n, int p, long long nsr)
+{
+    uint64_t h = mix64(tkey + (((uint64_t)n << 32) | (uint64_t)(uint32_t)p));
+    return (uint32_t)(((h >> 32) * (uint64_t)nsr) >> 32);
+}
+
+// ---------------------------------------------------------------------
+// exact reference pairwise sum (src/utils.c:29-45) over an element functor:
+// halves at n/2 until n <= 16, then sums sequentially.  One function per
+// depth level keeps the call graph acyclic.
+template <int D, class F>
+__device__ __noinline__ float pw_sum(const F &f, long long base, long long n)
+{
+    if (n <= 16 || D == 0) {
+        float s = 0.0f;
+        for (long long i = 0; i < n; ++i)
+            s += f(base + i);
+        return s;
+    }
+    long long h = n / 2;
+    if constexpr (D > 0)
+        return pw_sum<D - 1>(f, base, h) + pw_sum<D - 1>(f, base + h, n - h);
+    return 0.0f;
+}
+// depth 27 covers n up to 16 * 2^27
+template <class F>
+__device__ inline float pairwise(const F &f, long long base, long long n)
+{
+    return pw_sum<27>(f, base, n);
+}
+
+// ---------------------------------------------------------------------
+// (1) source-fit coefficients per (FSR, group) + FP64 accumulator seed.
+// Edge rule of src/solver.c:55-138 with row indices clamped into
+// [0, fai-1] (a no-op for fai >= 3; defined fallback for fai < 3).
+__global__ void k_prep(const float *__restrict__ src, const float *__restrict__ sig,
+                       const float *__restrict__ flux, float4 *__restrict__ coef,
+                       double *__restrict__ acc, long long nval, int G, int fai, int quad,
+                       float dz)
+{
+    const float two_dz = 2.f * dz;
+    const float two_dz2 = 2.f * dz * dz;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        long long fsr = e / G;
+        int g = (int)(e - fsr * G);
+        long long r = fsr / fai;
+        int fine = (int)(fsr - r * fai);
+        float s = sig[r * G + g];
+        float4 c;
+        if (quad) {
+            int r1, r2, r3;
+            if (fine == 0) {
+                r1 = 0; r2 = 1; r3 = 2;
+            } else if (fine == fai - 1) {
+                r1 = fine - 2; r2 = fine - 1; r3 = fine;
+            } else {
+                r1 = fine - 1; r2 = fine; r3 = fine + 1;
+            }
+            r1 = r1 < 0 ? 0 : (r1 > fai - 1 ? fai - 1 : r1);
+            r2 = r2 < 0 ? 0 : (r2 > fai - 1 ? fai - 1 : r2);
+            r3 = r3 > fai - 1 ? fai - 1 : r3;
+            const float *base = src + r * fai * G + g;
+            float y1 = base[(long long)r1 * G];
+            float y2 = base[(long long)r2 * G];
+            float y3 = base[(long long)r3 * G];
+            c.x = y2;
+            c.y = (y1 - y3) / two_dz;
+            c.z = (y1 - 2.f * y2 + y3) / two_dz2;
+        } else {
+            c.x = src[e];
+            c.y = 0.f;
+            c.z = 0.f;
+        }
+        c.w = s;
+        coef[e] = c;
+        if (acc)
+            acc[e] = (double)flux[e];
+    }
+}
+
+// ---------------------------------------------------------------------
+// (2) axial ray trace, one thread per z-stack (src/solver.c:366-527),
+// including the live [begin, end) update inside the k loop and the z reset
+// on every piece of a track's last 2D segment.
+template <bool WRITE>
+__global__ void k_trace(TraceArgs a)
+{
+    const long long nst = a.nt2 * a.np;
+    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < nst;
+         s += (long long)gridDim.x * blockDim.x) {
+        const long long i = s / a.np;
+        const int j = (int)(s - i * a.np);
+        const int zs = a.zs;
+        const long long t0 = s * zs;
+        const bool pos = j < a.np / 2;
+        const double c = a.cosd[j];
+        const double sn = a.sind[j];
+        float *z = WRITE ? a.z + t0 : a.zwork + t0;
+        for (int k = 0; k < zs; k++) {
+            if (!WRITE)
+                a.zwork[t0 + k] = a.z[t0 + k];
+            a.cnt[t0 + k] = 0;
+        }
+        int begin = 0, end = zs;
+        const long long off = a.seg_off[i];
+        const long long nseg = a.seg_off[i + 1] - off;
+        long long total = 0;
+        const long long sbase = (WRITE && a.qsr_stream) ? a.stack_off[s] : 0;
+        for (long long n = 0; n < nseg; n++) {
+            const float s_full = (float)((double)a.seg_len[off + n] / sn);
+            for (int k = begin; k < end; k++) {
+                const long long t = t0 + k;
+                float zk = z[k];
+                float sr = s_full;
+                int curr = pos ? (int)(zk / a.dzf) : (int)ceil((double)(zk / a.dzf));
+                bool complete = false;
+                int p = 0;
+                int cur = a.cnt[t];
+                uint64_t tkey = 0;
+                if (WRITE && !a.qsr_stream)
+                    tkey = track_key(a.sweep_key, t);
+                while (!complete) {
+                    bool reset = false;
+                    float ds;
+                    float zn = (float)((double)zk + (double)sr * c);
+                    int ni = pos ? (int)(zn / a.dzf) : (int)ceil((double)(zn / a.dzf));
+                    if (ni == curr) {
+                        complete = true;
+                        ds = sr;
+                    } else {
+                        curr += pos ? 1 : -1;
+                        zn = (float)(a.fine_dz * (double)(float)curr);
+                        ds = (float)((double)(zn - zk) / c);
+                        sr -= ds;
+                        if (sr <= 0.f)
+                            complete = true;
+                        if (zn <= 0.f || (double)zn >= a.node_dz) {
+                            complete = true;
+                            if (pos)
+                                end--;
+                            else
+                                begin++;
+                            reset = true;
+                        }
+                    }
+                    if (WRITE) {
+                        uint32_t qsr;
+                        if (a.qsr_stream) {
+                            int32_t v = a.qsr_stream[sbase + total];
+                            if (v < 0 || (long long)v >= a.nsr) {
+                                atomicOr(a.err, ERR_QSR_RANGE);
+                                v = 0;
+                            }
+                            qsr = (uint32_t)v;
+                        } else {
+                            qsr = piece_region(tkey, n, p, a.nsr);
+                        }
+                        // cell of the piece start, as attenuate_fluxes computes it
+                        const int cell = (int)(zk / a.dz_att);
+                        float zin = zk - a.dz_att * ((float)cell + 0.5f);
+                        int fine = cell % a.fai;
+                        if (a.quad) {
+                            if (fine == 0)
+                                zin -= a.dz_att;
+                            else if (fine == a.fai - 1)
+                                zin += a.dz_att;
+                        }
+                        if (fine < 0) {
+                            atomicOr(a.err, ERR_FINE_RANGE);
+                            fine = 0;
+                        }
+                        const long long q = a.off[t] + cur;
+                        if (q < a.rec_cap) {
+                            a.rec_ds[q] = ds;
+                            a.rec_zin[q] = zin;
+                            a.rec_fsr[q] = qsr * (uint32_t)a.fai + (uint32_t)fine;
+                        } else {
+                            atomicOr(a.err, ERR_RECORD_CAP);
+                        }
+                    }
+                    cur++;
+                    total++;
+                    p++;
+                    if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
+                        atomicOr(a.err, ERR_TRACE_RUNAWAY);
+                        complete = true;
+                    }
+                    if (n == nseg - 1 || reset)
+                        zk = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
+                    else
+                        zk = zn;
+                }
+                z[k] = zk;
+                a.cnt[t] = cur;
+            }
+        }
+        if (!WRITE && a.stack_cnt)
+            a.stack_cnt[s] = total;
+    }
+}
+
+// ---------------------------------------------------------------------
+// (3) attenuation (src/solver.c:14-280 quadratic, :1040-1138 flat)
+__device__ static inline float exp_term(const float2 *tab, float tau, const AttArgs &a)
+{
+    if (a.exp_mode != MOC_EXP_TABLE)
+        return -expm1f(-tau);
+    if (tau > a.maxv)
+        return 1.0f;
+    float v = tau / a.dx + a.hdx;
+    int iv;
+    if (!(v > -1.0f))
+        iv = 0;
+    else if (v >= (float)a.N)
+        iv = a.N - 1;
+    else
+        iv = (int)v;
+    float2 sl = tab[iv];
+    return sl.x * tau + sl.y;
+}
+
+template <int LPT, int GPL, bool QUAD, bool T64>
+__global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
+{
+    extern __shared__ float2 s_tab[];
+    for (int e = threadIdx.x; e < a.N; e += blockDim.x)
+        s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
+    __syncthreads();
+
+    constexpr int TPB = 256 / LPT;   // tracks per block
+    const int l = threadIdx.x % LPT;
+    const int G = a.G;
+    const long long nblk = (a.T + TPB - 1) / TPB;
+    for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const long long t = b * TPB + threadIdx.x / LPT;
+        if (t >= a.T)
+            continue;
+        const int j = (int)((t / a.zs) % a.np);
+        const long long i = t / ((long long)a.zs * a.np);
+        const float mu = a.mu[j];
+        const float mu2 = mu * mu;
+        const float azw = a.az[i];
+        const float weight = QUAD ? a.pw[t] * azw : a.pw[t] * azw * mu;
+        float psi[GPL];
+#pragma unroll
+        for (int r = 0; r < GPL; r++) {
+            const int g = l + r * LPT;
+            psi[r] = (g < G) ? a.psi[t * G + g] : 0.f;
+        }
+        const long long q0 = a.off[t], q1 = a.off[t + 1];
+        for (long long q = q0; q < q1; ++q) {
+            const float ds = a.rec_ds[q];
+            const float zin = a.rec_zin[q];
+            const uint32_t fsr = a.rec_fsr[q];
+            const float4 *cf = a.coef + (size_t)fsr * G;
+#pragma unroll
+            for (int r = 0; r < GPL; r++) {
+                const int g = l + r * LPT;
+                if (g >= G)
+                    continue;
+                const float4 c = cf[g];
+                const float sig = c.w;
+                const float tau = sig * ds;
+                const float F = exp_term(s_tab, tau, a);
+                const float ps = psi[r];
+                float tally;
+                if (QUAD) {
+                    const float q0v = c.x + c.y * zin + c.z * zin * zin;
+                    const float q1v = c.y + 2.f * c.z * zin;
+                    const float q2v = c.z;
+                    const float sig2 = sig * sig;
+                    const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * sig2);
+                    const float fi = (q0v * tau + (sig * ps - q0v) * F) / sig2 + q1v * mu * reuse +
+                                     q2v * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) /
+                                         (3.f * sig2 * sig2);
+                    tally = weight * fi;
+                    const float t1 = q0v * F / sig;
+                    const float t2 = q1v * mu * (tau - F) / sig2;
+                    const float t3 = q2v * mu2 * reuse;
+                    const float t4 = ps * (1.f - F);
+                    psi[r] = t1 + t2 + t3 + t4;
+                } else {
+                    const float qv = c.x / sig;
+                    const float dpsi = (ps - qv) * F;
+                    tally = weight * dpsi;
+                    psi[r] = ps - dpsi;
+                }
+                if (T64)
+                    atomicAdd(a.acc64 + (size_t)fsr * G + g, (double)tally);
+                else
+                    atomicAdd(a.flux32 + (size_t)fsr * G + g, tally);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < GPL; r++) {
+            const int g = l + r * LPT;
+            if (g < G)
+                a.psi[t * G + g] = psi[r];
+        }
+    }
+}
+
+__global__ void k_finalize(const double *__restrict__ acc, float *__restrict__ flux, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        flux[e] = (float)acc[e];
+}
+
+// ---------------------------------------------------------------------
+// deterministic FP64 block reduction helpers
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double *out)
+{
+    __shared__ double sh[NV][1024 / 64];
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64, nw = blockDim.x / 64;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1)
+            x += __shfl_down(x, o, 64);
+        if (lane == 0)
+            sh[k][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int q = 0; q < nw; q++)
+                s += sh[k][q];
+            out[k] = s;
+        }
+    }
+}
+
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+// fission rate sum_{i,j,g} flux*vol*nuSigmaF (src/solver.c:1160-1175), FP64
+__global__ void k_fission_partial(const float *__restrict__ flux, const float *__restrict__ vol,
+                                  const int *__restrict__ mat, const float *__restrict__ xs,
+                                  long long nval, int G, int fai, double *partial)
+{
+    double v[1] = {0.0};
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        long long fsr = e / G;
+        int g = (int)(e - fsr * G);
+        long long r = fsr / fai;
+        v[0] += (double)(flux[e] * vol[r] * xs[((long long)mat[r] * G + g) * 3 + 0]);
+    }
+    block_sum_store<1>(v, partial + blockIdx.x);
+}
+
+// scalar slots in the device state block
+enum { SC_NF = 0, SC_LEAK = 1, SC_KEFF = 2, SC_RESID = 3, SC_BPSI_NAN = 4, SC_N = 8 };
+
+__global__ void k_finish_nf(const double *partial, int nb, float *scal)
+{
+    double v[1] = {0.0};
+    for (int q = threadIdx.x; q < nb; q += blockDim.x)
+        v[0] += partial[q];
+    __shared__ double tot[1];
+    block_sum_store<1>(v, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float nf = (float)(1.0 / tot[0]);
+        scal[SC_NF] = nf;
+        if (!isfinite(nf))
+            scal[SC_BPSI_NAN] = 1.0f;
+    }
+}
+
+// src/solver.c:1207-1214
+__global__ void k_scale_flux(float *__restrict__ flux, const float *__restrict__ vol,
+                             const float *__restrict__ scal, long long nval, int G, int fai)
+{
+    const float nf = scal[SC_NF];
+    const double base = (double)(nf * 4.0f) * M_PI * (double)fai;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        long long r = e / ((long long)G * fai);
+        float adjust = (float)(base / (double)vol[r]);
+        flux[e] *= adjust;
+    }
+}
+
+// src/solver.c:1217-1226
+__global__ void k_scale_psi(float *__restrict__ x, long long n, const float *__restrict__ scal)
+{
+    const float nf = scal[SC_NF];
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] *= nf;
+}
+
+// ---------------------------------------------------------------------
+// update_sources (src/solver.c:1254-1304): one thread per (FSR, group).
+// scT is the scattering matrix transposed per material: scT[m][g2][g].
+__global__ void k_update_sources(const float *__restrict__ flux, float *__restrict__ src,
+                                 const float *__restrict__ xs, const float *__restrict__ scT,
+                                 const int *__restrict__ mat, float *__restrict__ gres,
+                                 long long nval, int G, int fai, float inverse_k)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long fsr = e / G;
+        const int g = (int)(e - fsr * G);
+        const long long r = fsr / fai;
+        const long long m = mat[r];
+        const float *phi = flux + fsr * G;
+        const float *X = xs + m * G * 3;
+        const float *S = scT + m * (long long)G * G + g;
+        float fission = pairwise([&](long long q) { return phi[q] * X[q * 3]; }, 0, G);
+        fission *= inverse_k;
+        float scatter = pairwise([&](long long q) { return S[q * G] * phi[q]; }, 0, G);
+        float chi = X[g * 3 + 2];
+        float newSrc = (float)((double)(fission * chi + scatter) / (4.0 * M_PI));
+        float oldSrc = src[e];
+        gres[e] = (newSrc - oldSrc) * (newSrc - oldSrc) / (oldSrc * oldSrc);
+        src[e] = newSrc;
+    }
+}
+
+// residuals[i] = pairwise over fai of pairwise over G (src/solver.c:1301-1303)
+__global__ void k_residual_regions(const float *__restrict__ gres, float *__restrict__ rres,
+                                   long long nsr, int G, int fai)
+{
+    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nsr;
+         r += (long long)gridDim.x * blockDim.x) {
+        const float *base = gres + r * fai * G;
+        rres[r] = pairwise(
+            [&](long long j) {
+                const float *row = base + j * G;
+                return pairwise([&](long long g) { return row[g]; }, 0, G);
+            },
+            0, fai);
+    }
+}
+
+// Exact pairwise sum of v[0..n) by one block.  The nodes at depth D-1 of
+// the halving tree (D = first depth whose nodes all hold <= 16 values) are
+// evaluated in parallel; above that depth every node has two children, so
+// the remaining levels combine adjacent pairs in LDS.
+constexpr int PW_MAX_NODES = 4096;
+__global__ void k_pairwise_block(const float *__restrict__ v, long long n, float *out)
+{
+    __shared__ float node[PW_MAX_NODES];
+    if (n <= 16) {
+        if (threadIdx.x == 0) {
+            float s = 0.f;
+            for (long long i = 0; i < n; i++)
+                s += v[i];
+            *out = s;
+        }
+        return;
+    }
+    int D = 0;
+    while (((n + ((1LL << D) - 1)) >> D) > 16)
+        D++;
+    const int L = 1 << (D - 1);   // nodes at depth D-1
+    for (int m = threadIdx.x; m < L; m += blockDim.x) {
+        long long start = 0, len = n;
+        for (int b = D - 2; b >= 0; b--) {
+            long long h = len / 2;
+            if ((m >> b) & 1) {
+                start += h;
+                len -= h;
+            } else {
+                len = h;
+            }
+        }
+        float s;
+        if (len <= 16) {
+            s = 0.f;
+            for (long long q = 0; q < len; q++)
+                s += v[start + q];
+        } else {
+            long long h = len / 2;
+            float a0 = 0.f, a1 = 0.f;
+            for (long long q = 0; q < h; q++)
+                a0 += v[start + q];
+            for (long long q = h; q < len; q++)
+                a1 += v[start + q];
+            s = a0 + a1;
+        }
+        node[m] = s;
+    }
+    __syncthreads();
+    // every level halves the node count; each thread owns at most
+    // PW_MAX_NODES / (2 * blockDim.x) pairs, read before the barrier and
+    // written after it
+    constexpr int MAXP = PW_MAX_NODES / 2 / 1024;
+    for (int cnt = L; cnt > 1; cnt >>= 1) {
+        float part[MAXP];
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int m = threadIdx.x + u * (int)blockDim.x;
+            part[u] = (m < cnt / 2) ? node[2 * m] + node[2 * m + 1] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < MAXP; u++) {
+            const int m = threadIdx.x + u * (int)blockDim.x;
+            if (m < cnt / 2)
+                node[m] = part[u];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *out = node[0];
+}
+
+// ---------------------------------------------------------------------
+// k_eff (src/solver.c:1334-1425): FP64 absorption and fission sums
+__global__ void k_keff_partial(const float *__restrict__ flux, const int *__restrict__ mat,
+                               const float *__restrict__ xs, long long nval, int G, int fai,
+                               double *partial)
+{
+    double v[2] = {0.0, 0.0};
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        long long fsr = e / G;
+        int g = (int)(e - fsr * G);
+        long long r = fsr / fai;
+        const float *X = xs + ((long long)mat[r] * G + g) * 3;
+        v[0] += (double)(X[1] * flux[e]);
+        v[1] += (double)(X[0] * flux[e]);
+    }
+    block_sum_store<2>(v, partial + 2 * blockIdx.x);
+}
+
+__global__ void k_keff_final(const double *partial, int nb, float *scal)
+{
+    double v[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        v[0] += partial[2 * q];
+        v[1] += partial[2 * q + 1];
+    }
+    __shared__ double tot[2];
+    block_sum_store<2>(v, tot);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        scal[SC_KEFF] = (float)(tot[1] / (tot[0] + (double)scal[SC_LEAK]));
+}
+
+// vacuum boundary: FP64 sum of the exchanged psi range, then zeroing
+__global__ void k_sum_partial(const float *__restrict__ x, long long n, double *partial)
+{
+    double v[1] = {0.0};
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        v[0] += (double)x[e];
+    block_sum_store<1>(v, partial + blockIdx.x);
+}
+__global__ void k_add_leakage(const double *partial, int nb, float *scal)
+{
+    double v[1] = {0.0};
+    for (int q = threadIdx.x; q < nb; q += blockDim.x)
+        v[0] += partial[q];
+    __shared__ double tot[1];
+    block_sum_store<1>(v, tot);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        scal[SC_LEAK] = (float)((double)scal[SC_LEAK] + tot[0]);
+}
+__global__ void k_fill(float *__restrict__ x, long long n, float v)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] = v;
+}
+__global__ void k_fill_u32(unsigned *__restrict__ x, long long n, unsigned v)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] = v;
+}
+
+// ---------------------------------------------------------------------
+// context
+struct moc_ctx {
+    moc_options opt;
+    moc_input in;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // shape
+    long long nt2 = 0, T = 0, nsr = 0, nfsr = 0, nval = 0, n_mat = 0, n_seg2d = 0;
+    int G = 0, np = 0, zs = 0, fai = 0, cai = 0, decomp = 0, quad = 1;
+    int lpt = 64, gpl = 1;
+    // geometry
+    long long *d_segoff = nullptr;
+    float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
+    double *d_cosd = nullptr, *d_sind = nullptr;
+    float *d_pw = nullptr, *d_z = nullptr, *d_zwork = nullptr, *d_psi = nullptr, *d_bpsi = nullptr;
+    // sources
+    float *d_src = nullptr, *d_flux = nullptr, *d_sig = nullptr, *d_vol = nullptr;
+    int *d_mat = nullptr;
+    float *d_xs = nullptr, *d_scT = nullptr;
+    float4 *d_coef = nullptr;
+    double *d_acc = nullptr;
+    float *d_table = nullptr;
+    int table_N = 0;
+    float table_dx = 0, table_maxv = 0;
+    // trace
+    int *d_cnt = nullptr;
+    long long *d_off = nullptr, *d_stack_cnt = nullptr, *d_stack_off = nullptr;
+    float *d_rec_ds = nullptr, *d_rec_zin = nullptr;
+    uint32_t *d_rec_fsr = nullptr;
+    long long rec_cap = 0;
+    int32_t *d_stream = nullptr;
+    long long stream_cap = 0;
+    void *d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    // reductions
+    double *d_partial = nullptr;
+    float *d_gres = nullptr, *d_rres = nullptr;
+    float *d_scal = nullptr;
+    unsigned *d_err = nullptr;
+    // bookkeeping
+    long sweeps = 0;
+    long last_segments = 0;
+    moc_timing timing{};
+    hipEvent_t ev[8] = {};
+};
+
+static int launch_grid(long long work, int threads, int cap = 65536)
+{
+    long long b = (work + threads - 1) / threads;
+    if (b < 1)
+        b = 1;
+    if (b > cap)
+        b = cap;
+    return (int)b;
+}
+
+template <class Tp>
+static int dalloc(Tp **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(Tp));
+    if (e != hipSuccess)
+        return moc_set_error(MOC_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s",
+                             count * sizeof(Tp), hipGetErrorString(e));
+    return MOC_OK;
+}
+
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != MOC_OK)         \
+            return rc_;            \
+    } while (0)
+
+static void free_all(moc_ctx *c)
+{
+    void *ptrs[] = {c->d_segoff, c->d_seglen, c->d_az, c->d_mu, c->d_cosd, c->d_sind, c->d_pw,
+                    c->d_z, c->d_zwork, c->d_psi, c->d_bpsi, c->d_src, c->d_flux, c->d_sig,
+                    c->d_vol, c->d_mat, c->d_xs, c->d_scT, c->d_coef, c->d_acc, c->d_table,
+                    c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec_ds,
+                    c->d_rec_zin, c->d_rec_fsr, c->d_stream, c->d_scan_tmp, c->d_partial,
+                    c->d_gres, c->d_rres, c->d_scal, c->d_err};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    for (auto &e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" void moc_default_options(moc_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->qsr_mode = MOC_QSR_HASH;
+    o->seed = 1;
+    o->tally = MOC_TALLY_F64;
+    o->exp_mode = MOC_EXP_TABLE;
+    o->boundary = MOC_BC_REFLECTIVE;
+    o->keep_b_psi = 0;
+    o->lanes_per_track = 0;
+    o->device = 0;
+}
+
+extern "C" int moc_device_count(int *n)
+{
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess)
+        c = 0;
+    if (n)
+        *n = c;
+    return MOC_OK;
+}
+
+static int choose_launch(moc_ctx *c)
+{
+    int G = c->G;
+    int lpt = c->opt.lanes_per_track;
+    if (lpt == 0) {
+        lpt = 64;
+        if (G <= 8)
+            lpt = 8;
+        else if (G <= 16)
+            lpt = 16;
+        else if (G <= 32)
+            lpt = 32;
+    }
+    if (lpt != 8 && lpt != 16 && lpt != 32 && lpt != 64)
+        return moc_set_error(MOC_ERR_INVALID, "lanes_per_track must be 8, 16, 32 or 64");
+    int gpl = (G + lpt - 1) / lpt;
+    int gpl_p2 = 1;
+    while (gpl_p2 < gpl)
+        gpl_p2 *= 2;
+    if (gpl_p2 > 8 || (gpl_p2 > 1 && lpt != 64))
+        return moc_set_error(MOC_ERR_INVALID,
+                             "n_egroups %d needs %d groups per lane with %d lanes per track "
+                             "(supported: up to 8 groups per lane at 64 lanes, 1 otherwise)",
+                             G, gpl, lpt);
+    c->lpt = lpt;
+    c->gpl = gpl_p2;
+    return MOC_OK;
+}
+
+extern "C" int moc_create(const moc_problem *p, const moc_options *o_in, moc_ctx **out)
+{
+    if (!p || !out)
+        return moc_set_error(MOC_ERR_INVALID, "moc_create: null argument");
+    if (!p->in.derived)
+        return moc_set_error(MOC_ERR_INVALID, "moc_create: problem inputs not derived");
+    moc_options o;
+    if (o_in)
+        o = *o_in;
+    else
+        moc_default_options(&o);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return moc_set_error(MOC_ERR_DEVICE, "no HIP device available (this library needs a gfx950 GPU)");
+    if (o.device < 0 || o.device >= ndev)
+        return moc_set_error(MOC_ERR_DEVICE, "device %d out of range (%d devices)", o.device, ndev);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, o.device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return moc_set_error(MOC_ERR_DEVICE, "device %d is %s; this library is built for gfx950",
+                             o.device, prop.gcnArchName);
+    HIP_TRY(hipSetDevice(o.device));
+
+    moc_ctx *c = new moc_ctx();
+    c->opt = o;
+    c->in = p->in;
+    c->device = o.device;
+    const moc_input &I = p->in;
+    c->nt2 = I.ntracks_2D;
+    c->np = I.n_polar_angles;
+    c->zs = I.z_stacked;
+    c->T = I.ntracks;
+    c->G = I.n_egroups;
+    c->nsr = I.n_source_regions_per_node;
+    c->fai = I.fai;
+    c->cai = I.cai;
+    c->decomp = I.decomp_assemblies_ax;
+    c->quad = I.axial_exp == 2;
+    c->nfsr = c->nsr * c->fai;
+    c->nval = c->nfsr * c->G;
+    c->n_mat = p->n_mat;
+    c->n_seg2d = p->n_seg2d;
+    c->table_N = p->table_N;
+    c->table_dx = p->table_dx;
+    c->table_maxv = p->table_maxVal;
+
+    auto fail = [&](int rc) {
+        free_all(c);
+        delete c;
+        return rc;
+    };
+    if (I.axial_exp != 0 && I.axial_exp != 2)
+        return fail(moc_set_error(MOC_ERR_INVALID, "invalid axial expansion order %d", I.axial_exp));
+    if ((unsigned long long)c->nsr * (unsigned long long)c->fai >= (1ull << 32))
+        return fail(moc_set_error(MOC_ERR_INVALID, "too many flat source regions"));
+    // the residual's pairwise tree is reduced by one block (k_pairwise_block)
+    if (c->nsr > 16LL * PW_MAX_NODES * 2)
+        return fail(moc_set_error(MOC_ERR_INVALID, "%lld source regions exceed the %lld supported",
+                                  c->nsr, 16LL * PW_MAX_NODES * 2));
+    if ((size_t)c->table_N * sizeof(float2) > 65536)
+        return fail(moc_set_error(MOC_ERR_INVALID, "exponential table of %d cells exceeds 64 KiB of LDS",
+                                  c->table_N));
+    int rc = choose_launch(c);
+    if (rc != MOC_OK)
+        return fail(rc);
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(moc_set_error(MOC_ERR_HIP, "hipStreamCreate failed"));
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess)
+            return fail(moc_set_error(MOC_ERR_HIP, "hipEventCreate failed"));
+
+    const long long nt2 = c->nt2, T = c->T, G = c->G, nsr = c->nsr;
+    const long long nst = nt2 * c->np;
+    if ((rc = dalloc(&c->d_segoff, nt2 + 1)) || (rc = dalloc(&c->d_seglen, c->n_seg2d)) ||
+        (rc = dalloc(&c->d_az, nt2)) || (rc = dalloc(&c->d_mu, c->np)) ||
+        (rc = dalloc(&c->d_cosd, c->np)) || (rc = dalloc(&c->d_sind, c->np)) ||
+        (rc = dalloc(&c->d_pw, T)) || (rc = dalloc(&c->d_z, T)) || (rc = dalloc(&c->d_zwork, T)) ||
+        (rc = dalloc(&c->d_psi, (size_t)T * G)) || (rc = dalloc(&c->d_src, c->nval)) ||
+        (rc = dalloc(&c->d_flux, c->nval)) || (rc = dalloc(&c->d_sig, nsr * G)) ||
+        (rc = dalloc(&c->d_vol, nsr)) || (rc = dalloc(&c->d_mat, nsr)) ||
+        (rc = dalloc(&c->d_xs, c->n_mat * G * 3)) || (rc = dalloc(&c->d_scT, c->n_mat * G * G)) ||
+        (rc = dalloc(&c->d_coef, c->nval)) || (rc = dalloc(&c->d_table, 2 * (long long)c->table_N)) ||
+        (rc = dalloc(&c->d_cnt, T + 1)) || (rc = dalloc(&c->d_off, T + 1)) ||
+        (rc = dalloc(&c->d_stack_cnt, nst + 1)) || (rc = dalloc(&c->d_stack_off, nst + 1)) ||
+        (rc = dalloc(&c->d_partial, 2 * RED_BLOCKS)) || (rc = dalloc(&c->d_gres, c->nval)) ||
+        (rc = dalloc(&c->d_rres, nsr)) || (rc = dalloc(&c->d_scal, SC_N)) ||
+        (rc = dalloc(&c->d_err, 1)))
+        return fail(rc);
+    if (o.tally == MOC_TALLY_F64 && (rc = dalloc(&c->d_acc, c->nval)))
+        return fail(rc);
+    if (o.keep_b_psi && (rc = dalloc(&c->d_bpsi, (size_t)T * G)))
+        return fail(rc);
+
+    // host-side conversions
+    std::vector<long long> segoff(nt2 + 1);
+    for (long long i = 0; i <= nt2; i++)
+        segoff[i] = p->seg_offset[i];
+    std::vector<float> mu(c->np);
+    std::vector<double> cosd(c->np), sind(c->np);
+    for (int j = 0; j < c->np; j++) {
+        cosd[j] = cos((double)p->polar_angle[j]);
+        sind[j] = sin((double)p->polar_angle[j]);
+        mu[j] = (float)cosd[j];
+    }
+    // scattering transposed per material: scT[m][g2][g] = scatter[m][g][g2]
+    std::vector<float> scT((size_t)c->n_mat * G * G);
+    for (long long m = 0; m < c->n_mat; m++)
+        for (long long g = 0; g < G; g++)
+            for (long long g2 = 0; g2 < G; g2++)
+                scT[(m * G + g2) * G + g] = p->scatter[(m * G + g) * G + g2];
+    std::vector<float> scal(SC_N, 0.0f);
+    scal[SC_LEAK] = p->leakage;
+
+    auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes == 0)
+            return MOC_OK;
+        hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return moc_set_error(MOC_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+        return MOC_OK;
+    };
+    if ((rc = up(c->d_segoff, segoff.data(), (nt2 + 1) * 8)) ||
+        (rc = up(c->d_seglen, p->seg_length, c->n_seg2d * 4)) || (rc = up(c->d_az, p->az_weight, nt2 * 4)) ||
+        (rc = up(c->d_mu, mu.data(), c->np * 4)) || (rc = up(c->d_cosd, cosd.data(), c->np * 8)) ||
+        (rc = up(c->d_sind, sind.data(), c->np * 8)) || (rc = up(c->d_pw, p->p_weight, T * 4)) ||
+        (rc = up(c->d_z, p->z_height, T * 4)) || (rc = up(c->d_src, p->fine_source, c->nval * 4)) ||
+        (rc = up(c->d_flux, p->fine_flux, c->nval * 4)) || (rc = up(c->d_sig, p->sigT, nsr * G * 4)) ||
+        (rc = up(c->d_vol, p->vol, nsr * 4)) || (rc = up(c->d_mat, p->material, nsr * 4)) ||
+        (rc = up(c->d_xs, p->xs, c->n_mat * G * 3 * 4)) ||
+        (rc = up(c->d_scT, scT.data(), scT.size() * 4)) ||
+        (rc = up(c->d_table, p->table, 2 * (size_t)c->table_N * 4)) ||
+        (rc = up(c->d_scal, scal.data(), SC_N * 4)))
+        return fail(rc);
+    if (p->f_psi)
+        rc = up(c->d_psi, p->f_psi, (size_t)T * G * 4);
+    else if (hipMemset(c->d_psi, 0, (size_t)T * G * 4) != hipSuccess)
+        rc = moc_set_error(MOC_ERR_HIP, "hipMemset psi failed");
+    if (rc)
+        return fail(rc);
+    if (c->d_bpsi) {
+        if (p->b_psi)
+            rc = up(c->d_bpsi, p->b_psi, (size_t)T * G * 4);
+        else if (hipMemset(c->d_bpsi, 0, (size_t)T * G * 4) != hipSuccess)
+            rc = moc_set_error(MOC_ERR_HIP, "hipMemset b_psi failed");
+        if (rc)
+            return fail(rc);
+    }
+    if (hipMemset(c->d_err, 0, 4) != hipSuccess || hipMemset(c->d_cnt, 0, (T + 1) * 4) != hipSuccess)
+        return fail(moc_set_error(MOC_ERR_HIP, "hipMemset failed"));
+    // scan temporary storage (largest of the two scans)
+    size_t b1 = 0, b2 = 0;
+    if (rocprim::exclusive_scan(nullptr, b1, c->d_cnt, c->d_off, (long long)0, (size_t)(T + 1),
+                                rocprim::plus<long long>(), c->stream) != hipSuccess ||
+        rocprim::exclusive_scan(nullptr, b2, c->d_stack_cnt, c->d_stack_off, (long long)0,
+                                (size_t)(nst + 1), rocprim::plus<long long>(), c->stream) != hipSuccess)
+        return fail(moc_set_error(MOC_ERR_HIP, "scan sizing failed"));
+    c->scan_tmp_bytes = b1 > b2 ? b1 : b2;
+    if (hipMalloc(&c->d_scan_tmp, c->scan_tmp_bytes) != hipSuccess)
+        return fail(moc_set_error(MOC_ERR_NOMEM, "scan storage allocation failed"));
+    if (hipDeviceSynchronize() != hipSuccess)
+        return fail(moc_set_error(MOC_ERR_HIP, "device synchronise failed after upload"));
+    *out = c;
+    return MOC_OK;
+}
+
+extern "C" int moc_destroy(moc_ctx *c)
+{
+    if (!c)
+        return MOC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return MOC_OK;
+}
+
+static int check_err(moc_ctx *c)
+{
+    unsigned err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, c->d_err, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err) {
+        unsigned zero = 0;
+        HIP_TRY(hipMemcpy(c->d_err, &zero, 4, hipMemcpyHostToDevice));
+        return moc_set_error(MOC_ERR_STATE, "device check failed (flags 0x%x: %s%s%s%s)", err,
+                             (err & ERR_QSR_RANGE) ? "region draw out of range " : "",
+                             (err & ERR_FINE_RANGE) ? "negative fine interval " : "",
+                             (err & ERR_RECORD_CAP) ? "record capacity exceeded " : "",
+                             (err & ERR_TRACE_RUNAWAY) ? "axial trace runaway" : "");
+    }
+    return MOC_OK;
+}
+
+static TraceArgs trace_args(moc_ctx *c)
+{
+    TraceArgs a;
+    memset(&a, 0, sizeof(a));
+    a.seg_off = c->d_segoff;
+    a.seg_len = c->d_seglen;
+    a.cosd = c->d_cosd;
+    a.sind = c->d_sind;
+    a.z = c->d_z;
+    a.zwork = c->d_zwork;
+    a.cnt = c->d_cnt;
+    a.stack_cnt = c->d_stack_cnt;
+    a.off = c->d_off;
+    a.stack_off = c->d_stack_off;
+    a.rec_ds = c->d_rec_ds;
+    a.rec_zin = c->d_rec_zin;
+    a.rec_fsr = c->d_rec_fsr;
+    a.rec_cap = c->rec_cap;
+    a.qsr_stream = nullptr;
+    a.err = c->d_err;
+    a.nt2 = c->nt2;
+    a.np = c->np;
+    a.zs = c->zs;
+    a.fai = c->fai;
+    a.quad = c->quad;
+    a.nsr = c->nsr;
+    // src/solver.c:288-289 and :38
+    a.node_dz = (double)(c->in.height / (float)c->decomp);
+    a.fine_dz = a.node_dz / (double)(c->cai * c->fai);
+    a.dzf = (float)a.fine_dz;
+    a.dz_att = c->in.height / (float)(c->fai * c->decomp * c->cai);
+    a.sep = c->in.axial_z_sep;
+    a.sweep_key = sweep_key(c->opt.seed, c->sweeps);
+    return a;
+}
+
+template <int LPT, int GPL>
+static void launch_att_lpt(moc_ctx *c, const AttArgs &a, int grid, size_t lds)
+{
+    const bool t64 = c->opt.tally == MOC_TALLY_F64;
+    if (c->quad) {
+        if (t64)
+            hipLaunchKernelGGL((k_attenuate<LPT, GPL, true, true>), dim3(grid), dim3(256), lds, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_attenuate<LPT, GPL, true, false>), dim3(grid), dim3(256), lds, c->stream, a);
+    } else {
+        if (t64)
+            hipLaunchKernelGGL((k_attenuate<LPT, GPL, false, true>), dim3(grid), dim3(256), lds, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_attenuate<LPT, GPL, false, false>), dim3(grid), dim3(256), lds, c->stream, a);
+    }
+}
+
+static void launch_attenuate(moc_ctx *c, const AttArgs &a)
+{
+    const int tpb = 256 / c->lpt;
+    const long long nblk = (c->T + tpb - 1) / tpb;
+    const int grid = (int)(nblk < 1048576 ? nblk : 1048576);
+    const size_t lds = (size_t)c->table_N * sizeof(float2);
+    switch (c->lpt) {
+    case 8: launch_att_lpt<8, 1>(c, a, grid, lds); break;
+    case 16: launch_att_lpt<16, 1>(c, a, grid, lds); break;
+    case 32: launch_att_lpt<32, 1>(c, a, grid, lds); break;
+    default:
+        switch (c->gpl) {
+        case 1: launch_att_lpt<64, 1>(c, a, grid, lds); break;
+        case 2: launch_att_lpt<64, 2>(c, a, grid, lds); break;
+        case 4: launch_att_lpt<64, 4>(c, a, grid, lds); break;
+        default: launch_att_lpt<64, 8>(c, a, grid, lds); break;
+        }
+    }
+}
+
+static int ensure_records(moc_ctx *c, long long S)
+{
+    if (S <= c->rec_cap)
+        return MOC_OK;
+    long long cap = S + S / 8 + 1024;
+    if (c->d_rec_ds) {
+        HIP_TRY(hipFree(c->d_rec_ds));
+        HIP_TRY(hipFree(c->d_rec_zin));
+        HIP_TRY(hipFree(c->d_rec_fsr));
+        c->d_rec_ds = nullptr;
+        c->d_rec_zin = nullptr;
+        c->d_rec_fsr = nullptr;
+        c->rec_cap = 0;
+    }
+    TRY(dalloc(&c->d_rec_ds, cap));
+    TRY(dalloc(&c->d_rec_zin, cap));
+    TRY(dalloc(&c->d_rec_fsr, cap));
+    c->rec_cap = cap;
+    return MOC_OK;
+}
+
+extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    const long long nst = c->nt2 * c->np;
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    // (1) coefficients + accumulator
+    hipLaunchKernelGGL(k_prep, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
+                       c->d_src, c->d_sig, c->d_flux, c->d_coef, c->d_acc, c->nval, c->G, c->fai,
+                       c->quad, c->in.height / (float)(c->fai * c->decomp * c->cai));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    // (2) count pass
+    TraceArgs ta = trace_args(c);
+    const int tgrid = launch_grid(nst, 64);
+    hipLaunchKernelGGL(k_trace<false>, dim3(tgrid), dim3(64), 0, c->stream, ta);
+    HIP_TRY(hipGetLastError());
+    // (3) offsets
+    size_t tb = c->scan_tmp_bytes;
+    HIP_TRY(rocprim::exclusive_scan(c->d_scan_tmp, tb, c->d_cnt, c->d_off, (long long)0,
+                                    (size_t)(c->T + 1), rocprim::plus<long long>(), c->stream));
+    long long S = 0;
+    HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    TRY(ensure_records(c, S));
+    ta.rec_ds = c->d_rec_ds;
+    ta.rec_zin = c->d_rec_zin;
+    ta.rec_fsr = c->d_rec_fsr;
+    ta.rec_cap = c->rec_cap;
+    if (c->opt.qsr_mode == MOC_QSR_LIBC) {
+        // region draws from the host's libc rand() in the reference's serial
+        // order (src/solver.c:481): stack-major, then the trace order
+        if (S > c->stream_cap) {
+            if (c->d_stream)
+                HIP_TRY(hipFree(c->d_stream));
+            c->d_stream = nullptr;
+            c->stream_cap = 0;
+            TRY(dalloc(&c->d_stream, S + 1));
+            c->stream_cap = S + 1;
+        }
+        std::vector<int32_t> draws((size_t)S);
+        for (long long q = 0; q < S; q++)
+            draws[q] = (int32_t)(rand() % c->nsr);
+        HIP_TRY(hipMemcpyAsync(c->d_stream, draws.data(), S * 4, hipMemcpyHostToDevice, c->stream));
+        size_t tb2 = c->scan_tmp_bytes;
+        HIP_TRY(rocprim::exclusive_scan(c->d_scan_tmp, tb2, c->d_stack_cnt, c->d_stack_off,
+                                        (long long)0, (size_t)nst, rocprim::plus<long long>(),
+                                        c->stream));
+        ta.qsr_stream = c->d_stream;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    // (4) write pass
+    hipLaunchKernelGGL(k_trace<true>, dim3(tgrid), dim3(64), 0, c->stream, ta);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    // (5) attenuation
+    AttArgs aa;
+    aa.off = c->d_off;
+    aa.rec_ds = c->d_rec_ds;
+    aa.rec_zin = c->d_rec_zin;
+    aa.rec_fsr = c->d_rec_fsr;
+    aa.coef = c->d_coef;
+    aa.table = c->d_table;
+    aa.psi = c->d_psi;
+    aa.acc64 = c->d_acc;
+    aa.flux32 = c->d_flux;
+    aa.pw = c->d_pw;
+    aa.az = c->d_az;
+    aa.mu = c->d_mu;
+    aa.T = c->T;
+    aa.G = c->G;
+    aa.np = c->np;
+    aa.zs = c->zs;
+    aa.N = c->table_N;
+    aa.dx = c->table_dx;
+    aa.hdx = 0.5f * c->table_dx;
+    aa.maxv = c->table_maxv;
+    aa.exp_mode = c->opt.exp_mode;
+    launch_attenuate(c, aa);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    // (6) flux = (float)accumulator
+    if (c->opt.tally == MOC_TALLY_F64) {
+        hipLaunchKernelGGL(k_finalize, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
+                           c->d_acc, c->d_flux, c->nval);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[4]));
+    TRY(check_err(c));
+    float ms[4];
+    for (int q = 0; q < 4; q++)
+        HIP_TRY(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+    c->timing.prep_ms = ms[0];
+    c->timing.trace_ms = ms[1];
+    c->timing.attenuate_ms = ms[2];
+    c->timing.finalize_ms = ms[3];
+    c->timing.sweep_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
+    c->last_segments = (long)S;
+    c->timing.segments = (long)S;
+    c->sweeps++;
+    c->timing.sweeps = c->sweeps;
+    if (segments_processed)
+        *segments_processed = (long)S;
+    return MOC_OK;
+}
+
+// src/comms.c:5-196 for one domain: the 6 faces have no neighbour, so each
+// face chunk (10,000 G-float blocks of the interleaved f/b buffer) is
+// tallied into leakage and replaced by the zero receive buffer.
+static long long vacuum_tracks(const moc_ctx *c)
+{
+    const long long tpm = 10000;
+    const float h = c->in.domain_height;
+    const float x = c->in.assembly_width;
+    const long long T = c->T;
+    long long ax = (long long)((float)T * x / (2 * x + 4 * h));
+    long long rad = (long long)((float)T * h / (2 * x + 4 * h));
+    long long rem = T - 2 * ax - 4 * rad;
+    long long add_rad = (long long)((float)rem * (4 * h / (2 * x + 4 * h)));
+    add_rad = 4 * (add_rad / 4);
+    rad += add_rad / 4;
+    long long add_ax = rem - add_rad;
+    ax += add_ax / 2;
+    long long chunks = 4 * (rad / tpm) + 2 * (ax / tpm);
+    // each chunk covers G*tpm floats of [T][2][G] = tpm/2 whole tracks
+    return chunks * tpm / 2;
+}
+
+extern "C" int moc_boundary_exchange(moc_ctx *c)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_boundary_exchange: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    if (c->opt.boundary == MOC_BC_REFLECTIVE) {
+        c->timing.boundary_ms = 0;
+        return MOC_OK;
+    }
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    long long nt = vacuum_tracks(c);
+    if (nt > c->T)
+        nt = c->T;
+    const long long n = nt * c->G;
+    hipLaunchKernelGGL(k_sum_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->d_psi, n,
+                       c->d_partial);
+    hipLaunchKernelGGL(k_add_leakage, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial,
+                       RED_BLOCKS, c->d_scal);
+    if (c->d_bpsi) {
+        hipLaunchKernelGGL(k_sum_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->d_bpsi,
+                           n, c->d_partial);
+        hipLaunchKernelGGL(k_add_leakage, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial,
+                           RED_BLOCKS, c->d_scal);
+    }
+    hipLaunchKernelGGL(k_fill, dim3(launch_grid(n, 256)), dim3(256), 0, c->stream, c->d_psi, n, 0.0f);
+    if (c->d_bpsi)
+        hipLaunchKernelGGL(k_fill, dim3(launch_grid(n, 256)), dim3(256), 0, c->stream, c->d_bpsi, n, 0.0f);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing.boundary_ms = ms;
+    return MOC_OK;
+}
+
+extern "C" int moc_renormalize_flux(moc_ctx *c)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_renormalize_flux: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_fission_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->d_flux,
+                       c->d_vol, c->d_mat, c->d_xs, c->nval, c->G, c->fai, c->d_partial);
+    hipLaunchKernelGGL(k_finish_nf, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial,
+                       RED_BLOCKS, c->d_scal);
+    hipLaunchKernelGGL(k_scale_flux, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
+                       c->d_flux, c->d_vol, c->d_scal, c->nval, c->G, c->fai);
+    const long long npsi = c->T * c->G;
+    hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_psi,
+                       npsi, c->d_scal);
+    if (c->d_bpsi)
+        hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream,
+                           c->d_bpsi, npsi, c->d_scal);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing.renorm_ms = ms;
+    return MOC_OK;
+}
+
+extern "C" int moc_update_sources(moc_ctx *c, float keff, float *residual)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_update_sources: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    const float inverse_k = (float)(1.0 / (double)keff);   // src/solver.c:1241
+    hipLaunchKernelGGL(k_update_sources, dim3(launch_grid(c->nval, 128)), dim3(128), 0, c->stream,
+                       c->d_flux, c->d_src, c->d_xs, c->d_scT, c->d_mat, c->d_gres, c->nval, c->G,
+                       c->fai, inverse_k);
+    hipLaunchKernelGGL(k_residual_regions, dim3(launch_grid(c->nsr, 64)), dim3(64), 0, c->stream,
+                       c->d_gres, c->d_rres, c->nsr, c->G, c->fai);
+    hipLaunchKernelGGL(k_pairwise_block, dim3(1), dim3(1024), 0, c->stream, c->d_rres, c->nsr,
+                       c->d_scal + SC_RESID);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    float res = 0;
+    HIP_TRY(hipMemcpyAsync(&res, c->d_scal + SC_RESID, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing.update_ms = ms;
+    if (residual)
+        *residual = res;
+    return MOC_OK;
+}
+
+extern "C" int moc_compute_keff(moc_ctx *c, float *keff)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_compute_keff: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    hipLaunchKernelGGL(k_keff_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->d_flux,
+                       c->d_mat, c->d_xs, c->nval, c->G, c->fai, c->d_partial);
+    hipLaunchKernelGGL(k_keff_final, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial,
+                       RED_BLOCKS, c->d_scal);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    float k = 0;
+    HIP_TRY(hipMemcpyAsync(&k, c->d_scal + SC_KEFF, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->timing.keff_ms = ms;
+    if (keff)
+        *keff = k;
+    return MOC_OK;
+}
+
+extern "C" int moc_download(moc_ctx *c, moc_problem *dst, unsigned what)
+{
+    if (!c || !dst)
+        return moc_set_error(MOC_ERR_INVALID, "moc_download: null argument");
+    if (dst->in.ntracks != c->T || dst->in.n_egroups != c->G ||
+        dst->in.n_source_regions_per_node != c->nsr || dst->in.fai != c->fai)
+        return moc_set_error(MOC_ERR_INVALID, "moc_download: problem shape differs from the context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((what & MOC_DL_FLUX) && dst->fine_flux)
+        HIP_TRY(hipMemcpy(dst->fine_flux, c->d_flux, c->nval * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_SOURCE) && dst->fine_source)
+        HIP_TRY(hipMemcpy(dst->fine_source, c->d_src, c->nval * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_PSI) && dst->f_psi)
+        HIP_TRY(hipMemcpy(dst->f_psi, c->d_psi, (size_t)c->T * c->G * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_Z) && dst->z_height)
+        HIP_TRY(hipMemcpy(dst->z_height, c->d_z, c->T * 4, hipMemcpyDeviceToHost));
+    float scal[SC_N];
+    HIP_TRY(hipMemcpy(scal, c->d_scal, SC_N * 4, hipMemcpyDeviceToHost));
+    if (what & MOC_DL_LEAKAGE)
+        dst->leakage = scal[SC_LEAK];
+    if ((what & MOC_DL_B_PSI) && dst->b_psi) {
+        if (c->d_bpsi) {
+            HIP_TRY(hipMemcpy(dst->b_psi, c->d_bpsi, (size_t)c->T * c->G * 4, hipMemcpyDeviceToHost));
+        } else {
+            // without a device copy b_psi is only ever scaled from zero:
+            // it stays 0, or NaN once a normalisation factor was not finite
+            const float v = scal[SC_BPSI_NAN] != 0.0f ? NAN : 0.0f;
+            for (long long e = 0; e < c->T * c->G; e++)
+                dst->b_psi[e] = v;
+        }
+    }
+    return MOC_OK;
+}
+
+extern "C" int moc_upload_state(moc_ctx *c, const moc_problem *src, unsigned what)
+{
+    if (!c || !src)
+        return moc_set_error(MOC_ERR_INVALID, "moc_upload_state: null argument");
+    if (src->in.ntracks != c->T || src->in.n_egroups != c->G ||
+        src->in.n_source_regions_per_node != c->nsr || src->in.fai != c->fai)
+        return moc_set_error(MOC_ERR_INVALID, "moc_upload_state: problem shape differs from the context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((what & MOC_DL_FLUX) && src->fine_flux)
+        HIP_TRY(hipMemcpy(c->d_flux, src->fine_flux, c->nval * 4, hipMemcpyHostToDevice));
+    if ((what & MOC_DL_SOURCE) && src->fine_source)
+        HIP_TRY(hipMemcpy(c->d_src, src->fine_source, c->nval * 4, hipMemcpyHostToDevice));
+    if ((what & MOC_DL_PSI) && src->f_psi)
+        HIP_TRY(hipMemcpy(c->d_psi, src->f_psi, (size_t)c->T * c->G * 4, hipMemcpyHostToDevice));
+    if ((what & MOC_DL_Z) && src->z_height)
+        HIP_TRY(hipMemcpy(c->d_z, src->z_height, c->T * 4, hipMemcpyHostToDevice));
+    if ((what & MOC_DL_B_PSI) && src->b_psi && c->d_bpsi)
+        HIP_TRY(hipMemcpy(c->d_bpsi, src->b_psi, (size_t)c->T * c->G * 4, hipMemcpyHostToDevice));
+    if (what & MOC_DL_LEAKAGE)
+        HIP_TRY(hipMemcpy(c->d_scal + SC_LEAK, &src->leakage, 4, hipMemcpyHostToDevice));
+    return MOC_OK;
+}
+
+extern "C" int moc_get_timing(moc_ctx *c, moc_timing *t)
+{
+    if (!c || !t)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_timing: null argument");
+    *t = c->timing;
+    return MOC_OK;
+}
+
+extern "C" int moc_get_launch(moc_ctx *c, int *lpt, int *gpl)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_launch: null context");
+    if (lpt)
+        *lpt = c->lpt;
+    if (gpl)
+        *gpl = c->gpl;
+    return MOC_OK;
+}
+
+extern "C" int moc_get_track_records(moc_ctx *c, long track, float *ds, float *zin, uint32_t *fsr,
+                                     long cap, long *n)
+{
+    if (!c || track < 0 || track >= c->T)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_track_records: bad track");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    long long o[2];
+    HIP_TRY(hipMemcpy(o, c->d_off + track, 16, hipMemcpyDeviceToHost));
+    long long cnt = o[1] - o[0];
+    if (n)
+        *n = (long)cnt;
+    long long m = cnt < cap ? cnt : cap;
+    if (m > 0) {
+        if (ds)
+            HIP_TRY(hipMemcpy(ds, c->d_rec_ds + o[0], m * 4, hipMemcpyDeviceToHost));
+        if (zin)
+            HIP_TRY(hipMemcpy(zin, c->d_rec_zin + o[0], m * 4, hipMemcpyDeviceToHost));
+        if (fsr)
+            HIP_TRY(hipMemcpy(fsr, c->d_rec_fsr + o[0], m * 4, hipMemcpyDeviceToHost));
+    }
+    return MOC_OK;
+}

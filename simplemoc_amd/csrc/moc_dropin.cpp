@@ -1,0 +1,327 @@
+// moc_dropin.cpp -- the reference's four solver symbols on top of the C ABI.
+//
+// transport_sweep / renormalize_flux / update_sources / compute_keff keep
+// the signatures of src/SimpleMOC_header.h:225-237.  On the first call the
+// reference's pointer structures are flattened into a moc_problem and a
+// device context is created; later calls reuse it while the Params
+// pointers and shape stay the same.  After each call the state the
+// reference function would have modified is written back into the
+// caller's arrays.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "moc_internal.h"
+#include "../../include/simplemoc_dropin.h"
+
+namespace {
+
+struct DropIn {
+    moc_ctx *ctx = nullptr;
+    moc_problem prob{};
+    // identity of the bound problem
+    const void *tracks_2D = nullptr;
+    const void *tracks = nullptr;
+    const void *sources = nullptr;
+    long ntracks = 0, nsr = 0;
+    int G = 0, fai = 0;
+    // owned flat storage
+    std::vector<float> az, seglen, polar, pw, z, psi, bpsi, scatter, xs, vol, src, flux, sig;
+    std::vector<long> segoff;
+    std::vector<int> mat;
+    bool sync_psi = true;
+};
+
+DropIn g_state;
+
+[[noreturn]] void die(const char *what)
+{
+    fprintf(stderr, "simplemoc drop-in: %s failed: %s\n", what, moc_last_error());
+    exit(1);
+}
+
+void release()
+{
+    if (g_state.ctx)
+        moc_destroy(g_state.ctx);
+    g_state = DropIn();
+}
+
+long env_long(const char *name, long dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? strtol(v, nullptr, 10) : dflt;
+}
+
+// flatten the reference's pointer structures (src/tracks.c:75-150,
+// src/source.c:97-204) into owned arrays and create the device context
+void bind(const moc_ref_Params *P, const moc_ref_Input *I)
+{
+    release();
+    DropIn &s = g_state;
+    const long nt2 = I->ntracks_2D, T = I->ntracks, nsr = I->n_source_regions_per_node;
+    const int G = I->n_egroups, fai = I->fai, np = I->n_polar_angles, zs = I->z_stacked;
+    s.tracks_2D = P->tracks_2D;
+    s.tracks = P->tracks;
+    s.sources = P->sources;
+    s.ntracks = T;
+    s.nsr = nsr;
+    s.G = G;
+    s.fai = fai;
+
+    moc_input in;
+    memset(&in, 0, sizeof(in));
+    in.x_assemblies = I->x_assemblies;
+    in.y_assemblies = I->y_assemblies;
+    in.cai = I->cai;
+    in.fai = fai;
+    in.axial_exp = I->axial_exp;
+    in.radial_ray_sep = I->radial_ray_sep;
+    in.axial_z_sep = I->axial_z_sep;
+    in.n_azimuthal = I->n_azimuthal;
+    in.n_polar_angles = np;
+    in.n_egroups = G;
+    in.decompose = I->decompose;
+    in.decomp_assemblies_ax = I->decomp_assemblies_ax;
+    in.segments_per_track = I->segments_per_track;
+    in.assembly_width = I->assembly_width;
+    in.height = I->height;
+    in.domain_height = I->domain_height;
+    in.precision = I->precision;
+    in.n_2D_source_regions_per_assembly = I->n_2D_source_regions_per_assembly;
+    in.papi_event_set = I->papi_event_set;
+    in.ntracks_2D = nt2;
+    in.z_stacked = zs;
+    in.ntracks = T;
+    in.n_source_regions_per_node = nsr;
+    in.derived = 1;
+
+    s.az.resize(nt2);
+    s.segoff.resize(nt2 + 1);
+    s.segoff[0] = 0;
+    for (long i = 0; i < nt2; i++) {
+        s.az[i] = P->tracks_2D[i].az_weight;
+        s.segoff[i + 1] = s.segoff[i] + P->tracks_2D[i].n_segments;
+    }
+    s.seglen.resize(s.segoff[nt2] + 1);
+    for (long i = 0; i < nt2; i++)
+        for (long n = 0; n < P->tracks_2D[i].n_segments; n++)
+            s.seglen[s.segoff[i] + n] = P->tracks_2D[i].segments[n].length;
+    s.polar.assign(P->polar_angles, P->polar_angles + np);
+
+    s.pw.resize(T);
+    s.z.resize(T);
+    s.psi.resize((size_t)T * G);
+    bool bpsi_nonzero = false;
+    for (long i = 0; i < nt2; i++)
+        for (int j = 0; j < np; j++)
+            for (int k = 0; k < zs; k++) {
+                const moc_ref_Track &tr = P->tracks[i][j][k];
+                long t = ((long)i * np + j) * zs + k;
+                s.pw[t] = tr.p_weight;
+                s.z[t] = tr.z_height;
+                memcpy(&s.psi[(size_t)t * G], tr.f_psi, G * sizeof(float));
+                for (int g = 0; g < G && !bpsi_nonzero; g++)
+                    bpsi_nonzero = tr.b_psi[g] != 0.0f;
+            }
+    if (bpsi_nonzero) {
+        s.bpsi.resize((size_t)T * G);
+        for (long t = 0; t < T; t++) {
+            long i = t / ((long)np * zs), j = (t / zs) % np, k = t % zs;
+            memcpy(&s.bpsi[(size_t)t * G], P->tracks[i][j][k].b_psi, G * sizeof(float));
+        }
+    }
+
+    // materials: sources sharing the (XS, scattering) row tables
+    std::map<std::pair<const void *, const void *>, int> mats;
+    s.mat.resize(nsr);
+    std::vector<long> first;
+    for (long r = 0; r < nsr; r++) {
+        auto key = std::make_pair((const void *)P->sources[r].XS,
+                                  (const void *)P->sources[r].scattering_matrix);
+        auto it = mats.find(key);
+        if (it == mats.end()) {
+            it = mats.emplace(key, (int)first.size()).first;
+            first.push_back(r);
+        }
+        s.mat[r] = it->second;
+    }
+    const long nm = (long)first.size();
+    s.xs.resize((size_t)nm * G * 3);
+    s.scatter.resize((size_t)nm * G * G);
+    for (long m = 0; m < nm; m++) {
+        const moc_ref_Source &sr = P->sources[first[m]];
+        for (int g = 0; g < G; g++) {
+            for (int c = 0; c < 3; c++)
+                s.xs[((size_t)m * G + g) * 3 + c] = sr.XS[g][c];
+            memcpy(&s.scatter[((size_t)m * G + g) * G], sr.scattering_matrix[g], G * sizeof(float));
+        }
+    }
+    s.vol.resize(nsr);
+    s.src.resize((size_t)nsr * fai * G);
+    s.flux.resize((size_t)nsr * fai * G);
+    s.sig.resize((size_t)nsr * G);
+    for (long r = 0; r < nsr; r++) {
+        const moc_ref_Source &sr = P->sources[r];
+        s.vol[r] = sr.vol;
+        memcpy(&s.sig[(size_t)r * G], sr.sigT, G * sizeof(float));
+        for (int j = 0; j < fai; j++) {
+            memcpy(&s.src[((size_t)r * fai + j) * G], sr.fine_source[j], G * sizeof(float));
+            memcpy(&s.flux[((size_t)r * fai + j) * G], sr.fine_flux[j], G * sizeof(float));
+        }
+    }
+
+    moc_problem &p = s.prob;
+    memset(&p, 0, sizeof(p));
+    p.in = in;
+    p.n_seg2d = s.segoff[nt2];
+    p.n_mat = nm;
+    p.az_weight = s.az.data();
+    p.seg_offset = s.segoff.data();
+    p.seg_length = s.seglen.data();
+    p.polar_angle = s.polar.data();
+    p.p_weight = s.pw.data();
+    p.z_height = s.z.data();
+    p.f_psi = s.psi.data();
+    p.b_psi = s.bpsi.empty() ? nullptr : s.bpsi.data();
+    p.scatter = s.scatter.data();
+    p.xs = s.xs.data();
+    p.material = s.mat.data();
+    p.vol = s.vol.data();
+    p.fine_source = s.src.data();
+    p.fine_flux = s.flux.data();
+    p.sigT = s.sig.data();
+    p.leakage = P->leakage ? *P->leakage : 0.0f;
+    p.table_N = P->expTable.N;
+    p.table_dx = P->expTable.dx;
+    p.table_maxVal = P->expTable.maxVal;
+    p.table = P->expTable.values;
+
+    moc_options o;
+    moc_default_options(&o);
+    const char *q = getenv("SIMPLEMOC_QSR");
+    o.qsr_mode = (q && strcmp(q, "libc") == 0) ? MOC_QSR_LIBC : MOC_QSR_HASH;
+    o.seed = (uint64_t)env_long("SIMPLEMOC_SEED", 1);
+    o.device = (int)env_long("SIMPLEMOC_DEVICE", 0);
+    o.keep_b_psi = bpsi_nonzero ? 1 : 0;
+    s.sync_psi = env_long("SIMPLEMOC_DROPIN_SYNC_PSI", 1) != 0;
+    if (moc_create(&p, &o, &s.ctx) != MOC_OK)
+        die("device context creation");
+    // the table pointer belongs to the caller; the context holds its copy
+    p.table = nullptr;
+}
+
+void ensure(const moc_ref_Params *P, const moc_ref_Input *I)
+{
+    const DropIn &s = g_state;
+    if (!s.ctx || s.tracks_2D != P->tracks_2D || s.tracks != P->tracks || s.sources != P->sources ||
+        s.ntracks != I->ntracks || s.nsr != I->n_source_regions_per_node || s.G != I->n_egroups ||
+        s.fai != I->fai)
+        bind(P, I);
+}
+
+// copy device state back into the caller's structures
+void write_back(const moc_ref_Params *P, const moc_ref_Input *I, unsigned what)
+{
+    DropIn &s = g_state;
+    if (!s.sync_psi)
+        what &= ~(unsigned)MOC_DL_PSI;
+    if (moc_download(s.ctx, &s.prob, what) != MOC_OK)
+        die("download");
+    const int G = s.G, fai = s.fai, np = I->n_polar_angles, zs = I->z_stacked;
+    if (what & (MOC_DL_FLUX | MOC_DL_SOURCE)) {
+        for (long r = 0; r < s.nsr; r++)
+            for (int j = 0; j < fai; j++) {
+                if (what & MOC_DL_FLUX)
+                    memcpy(P->sources[r].fine_flux[j], &s.flux[((size_t)r * fai + j) * G],
+                           G * sizeof(float));
+                if (what & MOC_DL_SOURCE)
+                    memcpy(P->sources[r].fine_source[j], &s.src[((size_t)r * fai + j) * G],
+                           G * sizeof(float));
+            }
+    }
+    if (what & (MOC_DL_PSI | MOC_DL_Z | MOC_DL_B_PSI)) {
+        for (long i = 0; i < I->ntracks_2D; i++)
+            for (int j = 0; j < np; j++)
+                for (int k = 0; k < zs; k++) {
+                    moc_ref_Track &tr = P->tracks[i][j][k];
+                    long t = ((long)i * np + j) * zs + k;
+                    if (what & MOC_DL_Z)
+                        tr.z_height = s.z[t];
+                    if (what & MOC_DL_PSI)
+                        memcpy(tr.f_psi, &s.psi[(size_t)t * G], G * sizeof(float));
+                    if ((what & MOC_DL_B_PSI) && !s.bpsi.empty())
+                        memcpy(tr.b_psi, &s.bpsi[(size_t)t * G], G * sizeof(float));
+                }
+    }
+    if ((what & MOC_DL_LEAKAGE) && P->leakage)
+        *P->leakage = s.prob.leakage;
+}
+
+}  // namespace
+
+extern "C" void moc_dropin_reset(void) { release(); }
+
+// src/solver.c:283-552
+extern "C" void transport_sweep(moc_ref_Params *params, moc_ref_Input *I)
+{
+    if (I->mype == 0)
+        printf("Starting transport sweep ...\n");
+    if (I->axial_exp != 0 && I->axial_exp != 2) {
+        printf("Error: invalid axial expansion order");
+        printf("\n Please input 0 or 2\n");
+        exit(1);
+    }
+    ensure(params, I);
+    long segs = 0;
+    if (moc_transport_sweep(g_state.ctx, &segs) != MOC_OK)
+        die("transport sweep");
+    write_back(params, I, MOC_DL_FLUX | MOC_DL_PSI | MOC_DL_Z);
+    I->segments_processed = segs;
+}
+
+// src/solver.c:1143-1230
+extern "C" void renormalize_flux(moc_ref_Params params, moc_ref_Input I, moc_ref_CommGrid grid)
+{
+    (void)grid;
+    if (I.mype == 0)
+        printf("Renormalizing Flux...\n");
+    ensure(&params, &I);
+    if (moc_renormalize_flux(g_state.ctx) != MOC_OK)
+        die("renormalize_flux");
+    unsigned what = MOC_DL_FLUX | MOC_DL_PSI;
+    if (!g_state.bpsi.empty())
+        what |= MOC_DL_B_PSI;
+    write_back(&params, &I, what);
+    if (I.mype == 0)
+        printf("Renormalizing Flux Complete.\n");
+}
+
+// src/solver.c:1235-1320
+extern "C" float update_sources(moc_ref_Params params, moc_ref_Input I, float keff)
+{
+    ensure(&params, &I);
+    float res = 0.0f;
+    if (moc_update_sources(g_state.ctx, keff, &res) != MOC_OK)
+        die("update_sources");
+    write_back(&params, &I, MOC_DL_SOURCE);
+    return res;
+}
+
+// src/solver.c:1324-1437 (non-MPI branch)
+extern "C" float compute_keff(moc_ref_Params params, moc_ref_Input I, moc_ref_CommGrid grid)
+{
+    (void)grid;
+    ensure(&params, &I);
+    // the caller owns the leakage tally
+    g_state.prob.leakage = params.leakage ? *params.leakage : 0.0f;
+    if (moc_upload_state(g_state.ctx, &g_state.prob, MOC_DL_LEAKAGE) != MOC_OK)
+        die("leakage upload");
+    float k = 0.0f;
+    if (moc_compute_keff(g_state.ctx, &k) != MOC_OK)
+        die("compute_keff");
+    return k;
+}

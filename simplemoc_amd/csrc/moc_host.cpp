@@ -1,0 +1,518 @@
+// moc_host.cpp -- host-side input handling and problem generation.
+//
+// Inputs follow the reference's Input handling (src/init.c, src/io.c);
+// MOC_GEN_REFERENCE reproduces build_tracks' libc rand() sequence
+// (src/init.c:106-159, src/tracks.c:4-168, src/source.c:4-214,
+// src/utils.c:4-26) so a problem generated here is bit-identical to the
+// reference's for the same srand seed.  MOC_GEN_SYNTHETIC draws the same
+// fields from a seeded counter-based generator with configurable
+// distributions (segment counts, lengths, sigma_t).
+#include "moc_internal.h"
+
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+// ---------------------------------------------------------------------
+// error reporting (thread-local message)
+static thread_local char g_err[512] = "";
+
+int moc_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *moc_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------
+// src/init.c:33-74
+extern "C" void moc_default_input(moc_input *I)
+{
+    memset(I, 0, sizeof(*I));
+    I->x_assemblies = 17;
+    I->y_assemblies = 17;
+    I->cai = 27;
+    I->fai = 5;
+    I->axial_exp = 2;
+    I->radial_ray_sep = 0.05f;
+    I->axial_z_sep = 0.25f;
+    I->n_azimuthal = 64;
+    I->n_polar_angles = 10;
+    I->n_egroups = 104;
+    I->decompose = 1;
+    I->decomp_assemblies_ax = 20;
+    I->segments_per_track = 120;
+    I->assembly_width = 21.42f;
+    I->height = 400.0f;
+    I->precision = 0.01f;
+    I->n_2D_source_regions_per_assembly = 5000;
+}
+
+// src/init.c:77-103
+extern "C" void moc_small_input(moc_input *I)
+{
+    I->x_assemblies = 15;
+    I->y_assemblies = 15;
+    I->cai = 5;
+    I->fai = 3;
+    I->axial_exp = 2;
+    I->radial_ray_sep = 0.5f;
+    I->axial_z_sep = 0.2f;
+    I->n_azimuthal = 5;
+    I->n_polar_angles = 5;
+    I->n_egroups = 104;
+    I->decompose = 0;
+    I->decomp_assemblies_ax = 1;
+    I->segments_per_track = 120;
+    I->assembly_width = (float)(1.26 * 17);
+    I->height = 400.0f;
+    I->precision = 0.01f;
+    I->n_2D_source_regions_per_assembly = 3000;
+}
+
+// src/io.c:198-270: 18 values, each followed by the rest of its line.
+static bool next_value(FILE *fp, char *buf, size_t n)
+{
+    // skip blank lines; the value is the first token of the next line
+    while (fgets(buf, (int)n, fp)) {
+        char *p = buf;
+        while (*p == ' ' || *p == '\t')
+            p++;
+        if (*p == '\n' || *p == '\r' || *p == 0)
+            continue;
+        memmove(buf, p, strlen(p) + 1);
+        return true;
+    }
+    return false;
+}
+
+extern "C" int moc_read_input_file(const char *path, moc_input *I)
+{
+    if (!path || !I)
+        return moc_set_error(MOC_ERR_INVALID, "moc_read_input_file: null argument");
+    FILE *fp = fopen(path, "r");
+    if (!fp)
+        return moc_set_error(MOC_ERR_IO, "cannot open input file '%s': %s", path, strerror(errno));
+    char line[1024];
+    long iv[18];
+    float fv[18];
+    // field kinds in order: i = int/long, f = float
+    static const char kinds[] = "iiiiiffiiiiiifffii";
+    int field = 0;
+    for (field = 0; field < 18; field++) {
+        if (!next_value(fp, line, sizeof(line))) {
+            fclose(fp);
+            return moc_set_error(MOC_ERR_IO, "input file '%s': expected 18 values, found %d",
+                                 path, field);
+        }
+        char *end = nullptr;
+        errno = 0;
+        if (kinds[field] == 'f') {
+            fv[field] = strtof(line, &end);
+        } else {
+            iv[field] = strtol(line, &end, 10);
+        }
+        if (end == line || errno != 0) {
+            fclose(fp);
+            return moc_set_error(MOC_ERR_IO, "input file '%s': value %d is not a number: %s",
+                                 path, field + 1, line);
+        }
+    }
+    fclose(fp);
+    I->x_assemblies = (int)iv[0];
+    I->y_assemblies = (int)iv[1];
+    I->cai = (int)iv[2];
+    I->fai = (int)iv[3];
+    I->axial_exp = (int)iv[4];
+    I->radial_ray_sep = fv[5];
+    I->axial_z_sep = fv[6];
+    I->n_azimuthal = (int)iv[7];
+    I->n_polar_angles = (int)iv[8];
+    I->n_egroups = (int)iv[9];
+    I->decompose = iv[10] == 0 ? 0 : 1;
+    I->decomp_assemblies_ax = (int)iv[11];
+    I->segments_per_track = iv[12];
+    I->assembly_width = fv[13];
+    I->height = fv[14];
+    I->precision = fv[15];
+    I->n_2D_source_regions_per_assembly = iv[16];
+    I->papi_event_set = (int)iv[17];
+    I->derived = 0;
+    return MOC_OK;
+}
+
+// src/init.c:4-30 (floating-point types as written there)
+extern "C" int moc_derive_inputs(moc_input *I)
+{
+    if (!I)
+        return moc_set_error(MOC_ERR_INVALID, "moc_derive_inputs: null input");
+    if (I->derived)
+        return moc_set_error(MOC_ERR_INVALID, "moc_derive_inputs: inputs already derived");
+    if (I->n_azimuthal < 2 || I->n_polar_angles < 1 || I->n_egroups < 1 || I->cai < 1 ||
+        I->fai < 1 || I->decomp_assemblies_ax < 1 || !(I->radial_ray_sep > 0) ||
+        !(I->axial_z_sep > 0) || !(I->height > 0) || !(I->assembly_width > 0) ||
+        !(I->precision > 0))
+        return moc_set_error(MOC_ERR_INVALID, "moc_derive_inputs: non-positive input value");
+    if (I->axial_exp != 0 && I->axial_exp != 2)
+        return moc_set_error(MOC_ERR_INVALID,
+                             "invalid axial expansion order %d: input 0 or 2", I->axial_exp);
+    I->n_azimuthal /= 2;
+    double per = (double)I->assembly_width * sqrt(2.0) / (double)I->radial_ray_sep;
+    long nt = (long)((double)I->n_azimuthal * per);
+    I->ntracks_2D = 2 * (nt / 2);
+    I->z_stacked = (int)(I->height / (I->axial_z_sep * (float)I->decomp_assemblies_ax));
+    I->ntracks = I->ntracks_2D * I->n_polar_angles * I->z_stacked;
+    I->domain_height = I->height / (float)I->decomp_assemblies_ax;
+    I->n_source_regions_per_node =
+        I->n_2D_source_regions_per_assembly * I->cai / I->decomp_assemblies_ax;
+    I->derived = 1;
+    if (I->ntracks_2D < 1 || I->z_stacked < 1 || I->n_source_regions_per_node < 1)
+        return moc_set_error(MOC_ERR_INVALID,
+                             "derived shape is empty (ntracks_2D %ld, z_stacked %d, regions %ld)",
+                             I->ntracks_2D, I->z_stacked, I->n_source_regions_per_node);
+    return MOC_OK;
+}
+
+// ---------------------------------------------------------------------
+// src/tracks.c:159-168
+extern "C" int moc_build_polar_angles(int n_polar, float *out)
+{
+    if (n_polar < 1 || !out)
+        return moc_set_error(MOC_ERR_INVALID, "moc_build_polar_angles: bad arguments");
+    for (int i = 0; i < n_polar; i++)
+        out[i] = (float)(M_PI * (i + 0.5) / n_polar);
+    return MOC_OK;
+}
+
+// src/utils.c:54
+extern "C" int moc_table_size(float precision, float max_val)
+{
+    return (int)((double)max_val * sqrt(1.0 / (8.0 * (double)precision * 0.01)));
+}
+
+// src/utils.c:48-78: slope -exp(-n dx) and intercept 1 + (n dx - 1) exp(-n dx)
+extern "C" int moc_build_table(float precision, float max_val, float *vals, int *N_out,
+                               float *dx_out, float *maxv_out)
+{
+    int N = moc_table_size(precision, max_val);
+    if (N < 1)
+        return moc_set_error(MOC_ERR_INVALID, "exponential table is empty (precision %g)",
+                             (double)precision);
+    float dx = max_val / (float)N;
+    if (vals) {
+        for (int n = 0; n < N; n++) {
+            float e = (float)exp((double)((float)(-n) * dx));
+            vals[2 * n] = -e;
+            vals[2 * n + 1] = 1.0f + ((float)n * dx - 1.0f) * e;
+        }
+    }
+    if (N_out)
+        *N_out = N;
+    if (dx_out)
+        *dx_out = dx;
+    if (maxv_out)
+        *maxv_out = max_val - dx;
+    return MOC_OK;
+}
+
+// ---------------------------------------------------------------------
+extern "C" void moc_gen_defaults(moc_gen_params *gp)
+{
+    memset(gp, 0, sizeof(*gp));
+    gp->mode = MOC_GEN_SYNTHETIC;
+    gp->seed = 1;
+    gp->seg_dist = MOC_SEGS_POISSON;
+    gp->seg_lo = 5;
+    gp->seg_hi = 500;
+    gp->len_lo = 0.01f;
+    gp->len_hi = 0.5f;
+    gp->sig_lo = 0.1f;
+    gp->sig_hi = 2.0f;
+    gp->alloc_psi = 0;
+    gp->alloc_b_psi = 0;
+}
+
+extern "C" void moc_problem_free(moc_problem *p)
+{
+    if (!p)
+        return;
+    free(p->az_weight);
+    free(p->seg_offset);
+    free(p->seg_length);
+    free(p->polar_angle);
+    free(p->p_weight);
+    free(p->z_height);
+    free(p->f_psi);
+    free(p->b_psi);
+    free(p->scatter);
+    free(p->xs);
+    free(p->material);
+    free(p->vol);
+    free(p->fine_source);
+    free(p->fine_flux);
+    free(p->sigT);
+    free(p->table);
+    free(p);
+}
+
+// --- libc-rand generators of the reference (src/utils.c:4-26) ---
+static float ref_urand(void) { return (float)rand() / (float)RAND_MAX; }
+
+static float ref_nrand(float mean, float sigma)
+{
+    float r1 = ref_urand();
+    float r2 = ref_urand();
+    float x = (float)(sqrt(-2 * log((double)r1)) * cos(2 * M_PI * (double)r2));
+    return x * sigma + mean;
+}
+
+// --- counter-based generator for the synthetic mode ---
+static inline uint64_t gen_mix(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+// streams
+enum : uint64_t { ST_AZW = 1, ST_NSEG, ST_LEN, ST_PW, ST_SCAT, ST_XS, ST_SRC, ST_SIG, ST_MAT, ST_VOL };
+
+static inline uint64_t gen_bits(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return gen_mix(gen_mix(seed ^ (stream * 0xD6E8FEB86659FD93ULL)) + idx * 0x9E3779B97F4A7C15ULL);
+}
+// uniform float in (0, 1): 24-bit midpoint lattice
+static inline float gen_u01(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return ((float)(gen_bits(seed, stream, idx) >> 40) + 0.5f) * (1.0f / 16777216.0f);
+}
+static inline double gen_u01d(uint64_t seed, uint64_t stream, uint64_t idx)
+{
+    return ((double)(gen_bits(seed, stream, idx) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+static long gen_poisson(double lambda, uint64_t seed, uint64_t idx)
+{
+    double u = gen_u01d(seed, ST_NSEG, idx);
+    if (lambda > 500.0) {
+        // normal approximation with a second draw (Box-Muller)
+        double u2 = gen_u01d(seed, ST_NSEG, idx ^ 0x8000000000000000ULL);
+        double z = sqrt(-2.0 * log(u)) * cos(2.0 * M_PI * u2);
+        long k = (long)floor(lambda + sqrt(lambda) * z + 0.5);
+        return k < 0 ? 0 : k;
+    }
+    double p = exp(-lambda);
+    double F = p;
+    long k = 0;
+    while (u > F && k < 100000) {
+        k++;
+        p *= lambda / (double)k;
+        F += p;
+    }
+    return k;
+}
+
+static int alloc_all(moc_problem *p, const moc_gen_params *gp)
+{
+    const moc_input &I = p->in;
+    const long nt2 = I.ntracks_2D, T = I.ntracks, G = I.n_egroups;
+    const long nsr = I.n_source_regions_per_node, fai = I.fai;
+    p->n_mat = nsr / 8 > 0 ? nsr / 8 : 1;
+    p->az_weight = (float *)malloc(nt2 * sizeof(float));
+    p->seg_offset = (long *)malloc((nt2 + 1) * sizeof(long));
+    p->polar_angle = (float *)malloc(I.n_polar_angles * sizeof(float));
+    p->p_weight = (float *)malloc(T * sizeof(float));
+    p->z_height = (float *)malloc(T * sizeof(float));
+    p->scatter = (float *)malloc((size_t)p->n_mat * G * G * sizeof(float));
+    p->xs = (float *)malloc((size_t)p->n_mat * G * 3 * sizeof(float));
+    p->material = (int *)malloc(nsr * sizeof(int));
+    p->vol = (float *)malloc(nsr * sizeof(float));
+    p->fine_source = (float *)malloc((size_t)nsr * fai * G * sizeof(float));
+    p->fine_flux = (float *)calloc((size_t)nsr * fai * G, sizeof(float));
+    p->sigT = (float *)malloc((size_t)nsr * G * sizeof(float));
+    if (gp->alloc_psi)
+        p->f_psi = (float *)calloc((size_t)T * G, sizeof(float));
+    if (gp->alloc_b_psi)
+        p->b_psi = (float *)calloc((size_t)T * G, sizeof(float));
+    if (!p->az_weight || !p->seg_offset || !p->polar_angle || !p->p_weight || !p->z_height ||
+        !p->scatter || !p->xs || !p->material || !p->vol || !p->fine_source || !p->fine_flux ||
+        !p->sigT || (gp->alloc_psi && !p->f_psi) || (gp->alloc_b_psi && !p->b_psi))
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for the problem arrays");
+    return MOC_OK;
+}
+
+static void init_z(moc_problem *p)
+{
+    // src/tracks.c:126-129
+    const moc_input &I = p->in;
+    for (long i = 0; i < I.ntracks_2D; i++)
+        for (int j = 0; j < I.n_polar_angles; j++)
+            for (int k = 0; k < I.z_stacked; k++) {
+                long t = ((long)i * I.n_polar_angles + j) * I.z_stacked + k;
+                p->z_height[t] = (j < I.n_polar_angles / 2) ? I.axial_z_sep * (float)k
+                                                            : I.axial_z_sep * (float)(k + 1);
+            }
+}
+
+static int gen_reference(moc_problem *p, const moc_gen_params *gp)
+{
+    const moc_input &I = p->in;
+    const long nt2 = I.ntracks_2D, G = I.n_egroups, nsr = I.n_source_regions_per_node;
+    const long fai = I.fai;
+    srand((unsigned)gp->seed);
+    // src/tracks.c:11-12
+    for (long i = 0; i < nt2; i++)
+        p->az_weight[i] = ref_urand();
+    // src/tracks.c:29-33 and :62-65
+    p->seg_offset[0] = 0;
+    const float mean = (float)I.segments_per_track;
+    const float sigma = (float)sqrt((double)I.segments_per_track);
+    for (long i = 0; i < nt2; i++) {
+        long ns = (long)ref_nrand(mean, sigma);
+        if (ns < 0)
+            return moc_set_error(MOC_ERR_INVALID, "negative segment count drawn for track %ld", i);
+        p->seg_offset[i + 1] = p->seg_offset[i] + ns;
+    }
+    p->n_seg2d = p->seg_offset[nt2];
+    p->seg_length = (float *)malloc((p->n_seg2d + 1) * sizeof(float));
+    if (!p->seg_length)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for segments");
+    // src/tracks.c:50-58
+    for (long i = 0; i < nt2; i++) {
+        long ns = p->seg_offset[i + 1] - p->seg_offset[i];
+        for (long n = 0; n < ns; n++)
+            p->seg_length[p->seg_offset[i] + n] = ref_urand() * I.assembly_width / (float)ns;
+    }
+    // src/tracks.c:117-132 (z heights and polar weights)
+    init_z(p);
+    for (long t = 0; t < I.ntracks; t++)
+        p->p_weight[t] = ref_urand();
+    // src/source.c:45-48, :83-86, :157-160, :169-172, :183-198
+    for (long e = 0; e < p->n_mat * G * G; e++)
+        p->scatter[e] = ref_urand();
+    for (long e = 0; e < p->n_mat * G * 3; e++)
+        p->xs[e] = ref_urand();
+    for (long e = 0; e < nsr * fai * G; e++)
+        p->fine_source[e] = ref_urand();
+    for (long e = 0; e < nsr * G; e++)
+        p->sigT[e] = ref_urand();
+    for (long i = 0; i < nsr; i++) {
+        p->material[i] = (i == 0) ? 0 : (int)(rand() % p->n_mat);
+        p->vol[i] = ref_urand();
+    }
+    return MOC_OK;
+}
+
+static int gen_synthetic(moc_problem *p, const moc_gen_params *gp)
+{
+    const moc_input &I = p->in;
+    const long nt2 = I.ntracks_2D, G = I.n_egroups, nsr = I.n_source_regions_per_node;
+    const long fai = I.fai;
+    const uint64_t s = gp->seed;
+    if (!(gp->len_hi >= gp->len_lo) || !(gp->len_lo > 0) || !(gp->sig_hi >= gp->sig_lo) ||
+        !(gp->sig_lo > 0))
+        return moc_set_error(MOC_ERR_INVALID, "bad synthetic distribution bounds");
+    for (long i = 0; i < nt2; i++)
+        p->az_weight[i] = gen_u01(s, ST_AZW, (uint64_t)i);
+    p->seg_offset[0] = 0;
+    for (long i = 0; i < nt2; i++) {
+        long ns;
+        if (gp->seg_dist == MOC_SEGS_UNIFORM) {
+            if (gp->seg_hi < gp->seg_lo || gp->seg_lo < 0)
+                return moc_set_error(MOC_ERR_INVALID, "bad segment-count bounds");
+            double u = gen_u01d(s, ST_NSEG, (uint64_t)i);
+            ns = gp->seg_lo + (long)(u * (double)(gp->seg_hi - gp->seg_lo + 1));
+            if (ns > gp->seg_hi)
+                ns = gp->seg_hi;
+        } else if (gp->seg_dist == MOC_SEGS_NORMAL) {
+            double u1 = gen_u01d(s, ST_NSEG, (uint64_t)i);
+            double u2 = gen_u01d(s, ST_NSEG, (uint64_t)i ^ 0x8000000000000000ULL);
+            double z = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+            double spt = (double)I.segments_per_track;
+            ns = (long)(spt + sqrt(spt) * z);
+            if (ns < 0)
+                ns = 0;
+        } else {
+            ns = gen_poisson((double)I.segments_per_track, s, (uint64_t)i);
+        }
+        p->seg_offset[i + 1] = p->seg_offset[i] + ns;
+    }
+    p->n_seg2d = p->seg_offset[nt2];
+    p->seg_length = (float *)malloc((p->n_seg2d + 1) * sizeof(float));
+    if (!p->seg_length)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for segments");
+    const float dl = gp->len_hi - gp->len_lo;
+    for (long e = 0; e < p->n_seg2d; e++)
+        p->seg_length[e] = gp->len_lo + dl * gen_u01(s, ST_LEN, (uint64_t)e);
+    init_z(p);
+    for (long t = 0; t < I.ntracks; t++)
+        p->p_weight[t] = gen_u01(s, ST_PW, (uint64_t)t);
+    for (long e = 0; e < p->n_mat * G * G; e++)
+        p->scatter[e] = gen_u01(s, ST_SCAT, (uint64_t)e);
+    for (long e = 0; e < p->n_mat * G * 3; e++)
+        p->xs[e] = gen_u01(s, ST_XS, (uint64_t)e);
+    for (long e = 0; e < nsr * fai * G; e++)
+        p->fine_source[e] = gen_u01(s, ST_SRC, (uint64_t)e);
+    const float ds = gp->sig_hi - gp->sig_lo;
+    for (long e = 0; e < nsr * G; e++)
+        p->sigT[e] = gp->sig_lo + ds * gen_u01(s, ST_SIG, (uint64_t)e);
+    for (long i = 0; i < nsr; i++) {
+        long m = (long)(gen_u01d(s, ST_MAT, (uint64_t)i) * (double)p->n_mat);
+        if (m >= p->n_mat)
+            m = p->n_mat - 1;
+        p->material[i] = (i == 0) ? 0 : (int)m;
+        p->vol[i] = gen_u01(s, ST_VOL, (uint64_t)i);
+    }
+    return MOC_OK;
+}
+
+extern "C" int moc_problem_generate(const moc_input *in, const moc_gen_params *gp_in,
+                                    moc_problem **out)
+{
+    if (!in || !out)
+        return moc_set_error(MOC_ERR_INVALID, "moc_problem_generate: null argument");
+    if (!in->derived)
+        return moc_set_error(MOC_ERR_INVALID, "moc_problem_generate: call moc_derive_inputs first");
+    moc_gen_params gp;
+    if (gp_in)
+        gp = *gp_in;
+    else
+        moc_gen_defaults(&gp);
+    moc_problem *p = (moc_problem *)calloc(1, sizeof(moc_problem));
+    if (!p)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed");
+    p->in = *in;
+    int rc = alloc_all(p, &gp);
+    if (rc == MOC_OK)
+        rc = moc_build_polar_angles(in->n_polar_angles, p->polar_angle);
+    if (rc == MOC_OK)
+        rc = (gp.mode == MOC_GEN_REFERENCE) ? gen_reference(p, &gp) : gen_synthetic(p, &gp);
+    if (rc == MOC_OK) {
+        int N = moc_table_size(in->precision, 10.0f);
+        p->table = (float *)malloc((size_t)2 * (N > 0 ? N : 1) * sizeof(float));
+        if (!p->table)
+            rc = moc_set_error(MOC_ERR_NOMEM, "host allocation failed for the table");
+        else
+            rc = moc_build_table(in->precision, 10.0f, p->table, &p->table_N, &p->table_dx,
+                                 &p->table_maxVal);
+    }
+    p->leakage = 0.0f;
+    if (rc != MOC_OK) {
+        moc_problem_free(p);
+        return rc;
+    }
+    *out = p;
+    return MOC_OK;
+}

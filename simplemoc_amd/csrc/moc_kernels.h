@@ -1,0 +1,85 @@
+// moc_kernels.h -- kernel argument blocks and launchers (device side of
+// the simplemoc_amd library).  Shared by moc_device.hip (context and C ABI)
+// and the kernel translation unit.
+#ifndef MOC_KERNELS_H
+#define MOC_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace moc {
+
+// Error bits raised by kernels into a device word.
+enum : unsigned {
+    ERR_QSR_RANGE = 1u,     // explicit region stream value out of range
+    ERR_FINE_RANGE = 2u,    // negative fine-interval index (never expected)
+    ERR_RECORD_CAP = 4u,    // record capacity exceeded (host sizing bug)
+    ERR_TRACE_RUNAWAY = 8u, // more than MAX_PIECES_PER_SEGMENT pieces in one segment
+};
+
+// Bound on the pieces of one (track, 2D segment); the axial trace ends far
+// below it for any input the reference can run.  Reaching it stops the
+// segment and raises ERR_TRACE_RUNAWAY instead of looping on.
+constexpr int MAX_PIECES_PER_SEGMENT = 1 << 20;
+
+// Axial ray trace of one (2D track, polar angle) z-stack per thread.
+struct TraceArgs {
+    const long long *seg_off;   // [nt2 + 1]
+    const float *seg_len;       // [n_seg2d]
+    const double *cosd;         // [np] cos(polar angle) in double
+    const double *sind;         // [np]
+    float *z;                   // [T] track z state (updated by the write pass)
+    float *zwork;               // [T] scratch copy for the count pass
+    int *cnt;                   // [T + 1] pieces per track (count) / cursor (write)
+    long long *stack_cnt;       // [nstacks] pieces per stack (count pass, may be null)
+    const long long *off;       // [T + 1] record offsets (write pass)
+    const long long *stack_off; // [nstacks] serial offsets of stacks (stream mode)
+    float *rec_ds;
+    float *rec_zin;
+    uint32_t *rec_fsr;
+    long long rec_cap;
+    const int32_t *qsr_stream;  // explicit region draws in serial order (or null)
+    unsigned *err;
+    long long nt2;
+    int np;
+    int zs;
+    int fai;
+    int quad;                   // axial_exp == 2
+    long long nsr;
+    double node_dz;             // height/decomp, float division widened (src/solver.c:288)
+    double fine_dz;             // node_dz/(cai*fai) in double (src/solver.c:289)
+    float dzf;                  // (float)fine_dz: interval functions take float dz
+    float dz_att;               // height/(fai*decomp*cai) in float (src/solver.c:38)
+    float sep;                  // axial_z_sep
+    uint64_t sweep_key;         // hash key of (seed, sweep)
+};
+
+// Attenuation: lanes of a track own groups g = lane + r*LPT.
+struct AttArgs {
+    const long long *off;
+    const float *rec_ds;
+    const float *rec_zin;
+    const uint32_t *rec_fsr;
+    const float4 *coef;         // [nfsr][G] {c0, c1, c2, sigma} or {src, 0, 0, sigma}
+    const float *table;         // [2N]
+    float *psi;                 // [T][G]
+    double *acc64;              // [nfsr][G] FP64 tally (or null)
+    float *flux32;              // [nfsr][G] FP32 tally target (or null)
+    const float *pw;            // [T]
+    const float *az;            // [nt2]
+    const float *mu;            // [np]
+    long long T;
+    int G;
+    int np;
+    int zs;
+    int N;
+    float dx;
+    float hdx;                  // 0.5f * dx
+    float maxv;
+    int exp_mode;
+};
+
+}  // namespace moc
+
+#endif

@@ -1,0 +1,254 @@
+/*
+ * simplemoc_main.c -- host driver of the gfx950 transport sweep.
+ *
+ * Keeps the reference's command line (src/io.c:115-181), its 18-field input
+ * file (src/io.c:198-270), the derived shape (src/init.c:4-30) and the
+ * per-iteration call sequence (src/main.c:57-92):
+ *     sweep -> [boundary exchange] -> renormalize -> update sources -> k_eff
+ * with per-phase wall-clock timers and the "Time per Intersection" metric
+ * (src/utils.c:147-155).  Every phase runs on the GPU through the C ABI of
+ * include/simplemoc.h.
+ *
+ * Options of the reference:
+ *   -t <threads>    accepted for compatibility (host threads are not used)
+ *   -i <file>       input file
+ *   -s              small problem (src/init.c:77-103)
+ *   -d <file>       OpenMOC track file: not supported (error)
+ * Added options:
+ *   -n <iters>      power iterations (reference: 1, src/main.c:41)
+ *   -S <seed>       problem seed (default 1)
+ *   -g synthetic|reference   problem generator (default synthetic)
+ *   -q hash|libc    region draws (default hash)
+ *   -b reflective|vacuum     one-domain boundary (default reflective)
+ *   -T f64|f32      tally accumulation (default f64)
+ *   -x table|intrinsic       1 - exp(-tau) evaluation (default table)
+ *   -D <device>     HIP device (default 0)
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "simplemoc.h"
+
+static double now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void usage(void)
+{
+    printf("Usage: ./simplemoc <options>\n");
+    printf("Options include:\n");
+    printf("  -t <threads>     Accepted for compatibility\n");
+    printf("  -i <filename>    Input file name\n");
+    printf("  -s               Small problem flag\n");
+    printf("  -n <iters>       Power iterations (default 1)\n");
+    printf("  -S <seed>        Problem seed (default 1)\n");
+    printf("  -g <generator>   synthetic | reference\n");
+    printf("  -q <draws>       hash | libc\n");
+    printf("  -b <boundary>    reflective | vacuum\n");
+    printf("  -T <tally>       f64 | f32\n");
+    printf("  -x <exp>         table | intrinsic\n");
+    printf("  -D <device>      HIP device index\n");
+    exit(1);
+}
+
+static void die(const char *what, int rc)
+{
+    fprintf(stderr, "simplemoc: %s failed (%d): %s\n", what, rc, moc_last_error());
+    exit(1);
+}
+
+static void border(void)
+{
+    for (int i = 0; i < 79; i++)
+        putchar('=');
+    putchar('\n');
+}
+
+static void center(const char *s)
+{
+    int pad = (79 - (int)strlen(s)) / 2;
+    printf("%*s%s\n", pad > 0 ? pad : 0, "", s);
+}
+
+int main(int argc, char *argv[])
+{
+    moc_input in;
+    moc_default_input(&in);
+    moc_gen_params gp;
+    moc_gen_defaults(&gp);
+    moc_options opt;
+    moc_default_options(&opt);
+    int iters = 1;
+    int rc;
+
+    for (int a = 1; a < argc; a++) {
+        const char *s = argv[a];
+        if (!strcmp(s, "-t")) {
+            if (++a >= argc)
+                usage();
+        } else if (!strcmp(s, "-i")) {
+            if (++a >= argc)
+                usage();
+            if ((rc = moc_read_input_file(argv[a], &in)) != MOC_OK)
+                die("reading the input file", rc);
+        } else if (!strcmp(s, "-s")) {
+            moc_small_input(&in);
+        } else if (!strcmp(s, "-d")) {
+            fprintf(stderr, "simplemoc: OpenMOC track files (-d) are not supported\n");
+            return 1;
+        } else if (!strcmp(s, "-n")) {
+            if (++a >= argc)
+                usage();
+            iters = atoi(argv[a]);
+        } else if (!strcmp(s, "-S")) {
+            if (++a >= argc)
+                usage();
+            gp.seed = strtoull(argv[a], NULL, 10);
+        } else if (!strcmp(s, "-g")) {
+            if (++a >= argc)
+                usage();
+            gp.mode = !strcmp(argv[a], "reference") ? MOC_GEN_REFERENCE : MOC_GEN_SYNTHETIC;
+            if (gp.mode == MOC_GEN_REFERENCE)
+                gp.seg_dist = MOC_SEGS_NORMAL;
+        } else if (!strcmp(s, "-q")) {
+            if (++a >= argc)
+                usage();
+            opt.qsr_mode = !strcmp(argv[a], "libc") ? MOC_QSR_LIBC : MOC_QSR_HASH;
+        } else if (!strcmp(s, "-b")) {
+            if (++a >= argc)
+                usage();
+            opt.boundary = !strcmp(argv[a], "vacuum") ? MOC_BC_VACUUM : MOC_BC_REFLECTIVE;
+        } else if (!strcmp(s, "-T")) {
+            if (++a >= argc)
+                usage();
+            opt.tally = !strcmp(argv[a], "f32") ? MOC_TALLY_F32 : MOC_TALLY_F64;
+        } else if (!strcmp(s, "-x")) {
+            if (++a >= argc)
+                usage();
+            opt.exp_mode = !strcmp(argv[a], "intrinsic") ? MOC_EXP_INTRINSIC : MOC_EXP_TABLE;
+        } else if (!strcmp(s, "-D")) {
+            if (++a >= argc)
+                usage();
+            opt.device = atoi(argv[a]);
+        } else {
+            usage();
+        }
+    }
+    if (iters < 1)
+        usage();
+    if ((rc = moc_derive_inputs(&in)) != MOC_OK)
+        die("deriving the problem shape", rc);
+
+    border();
+    center("SimpleMOC transport sweep on AMD Instinct MI355X (gfx950)");
+    border();
+    center("INPUT SUMMARY");
+    border();
+    printf("%-40s%d x %d\n", "Assemblies:", in.x_assemblies, in.y_assemblies);
+    printf("%-40s%d\n", "Coarse Axial Intervals:", in.cai);
+    printf("%-40s%d\n", "Fine Axial Intervals per Coarse Region:", in.fai);
+    printf("%-40s%d\n", "Axial Source Expansion Order:", in.axial_exp);
+    printf("%-40s%.4f\n", "Radial Ray Separation:", in.radial_ray_sep);
+    printf("%-40s%.4f\n", "Axial Z Ray Separation:", in.axial_z_sep);
+    printf("%-40s%d\n", "Azimuthal Angles (one direction):", in.n_azimuthal);
+    printf("%-40s%d\n", "Polar Angles:", in.n_polar_angles);
+    printf("%-40s%d\n", "Energy Groups:", in.n_egroups);
+    printf("%-40s%d\n", "Axial Domains per Assembly:", in.decomp_assemblies_ax);
+    printf("%-40s%ld\n", "Segments per 2D Track (mean):", in.segments_per_track);
+    printf("%-40s%.2f / %.2f cm\n", "Assembly Width / Domain Height:", in.assembly_width,
+           in.domain_height);
+    printf("%-40s%ld\n", "2D Tracks:", in.ntracks_2D);
+    printf("%-40s%d\n", "Z-stacked Rays:", in.z_stacked);
+    printf("%-40s%ld\n", "3D Tracks:", in.ntracks);
+    printf("%-40s%ld\n", "Source Regions:", in.n_source_regions_per_node);
+    printf("%-40s%.1f MB\n", "Angular Flux (forward) on Device:",
+           (double)in.ntracks * in.n_egroups * 4 / 1e6);
+    border();
+    center("INITIALIZATION");
+    border();
+
+    double t0 = now();
+    moc_problem *prob = NULL;
+    if ((rc = moc_problem_generate(&in, &gp, &prob)) != MOC_OK)
+        die("generating the problem", rc);
+    double t1 = now();
+    moc_ctx *ctx = NULL;
+    if ((rc = moc_create(prob, &opt, &ctx)) != MOC_OK)
+        die("creating the device context", rc);
+    double t2 = now();
+    printf("Problem generated in %.2f s, uploaded in %.2f s\n", t1 - t0, t2 - t1);
+    border();
+    center("SIMULATION");
+    border();
+
+    double t_transport = 0, t_exchange = 0, t_renorm = 0, t_update = 0, t_keff = 0;
+    double dev_sweep_ms = 0;
+    long total_segments = 0;
+    float keff = 1.0f, res = 0.0f;
+    for (int it = 0; it < iters; it++) {
+        long segs = 0;
+        double s = now();
+        if ((rc = moc_transport_sweep(ctx, &segs)) != MOC_OK)
+            die("transport sweep", rc);
+        double e = now();
+        t_transport += e - s;
+        total_segments += segs;
+        moc_timing tm;
+        moc_get_timing(ctx, &tm);
+        dev_sweep_ms += tm.sweep_ms;
+
+        s = now();
+        if ((rc = moc_boundary_exchange(ctx)) != MOC_OK)
+            die("boundary exchange", rc);
+        e = now();
+        t_exchange += e - s;
+
+        s = now();
+        if ((rc = moc_renormalize_flux(ctx)) != MOC_OK)
+            die("renormalize_flux", rc);
+        e = now();
+        t_renorm += e - s;
+
+        s = now();
+        if ((rc = moc_update_sources(ctx, keff, &res)) != MOC_OK)
+            die("update_sources", rc);
+        e = now();
+        t_update += e - s;
+
+        s = now();
+        if ((rc = moc_compute_keff(ctx, &keff)) != MOC_OK)
+            die("compute_keff", rc);
+        e = now();
+        t_keff += e - s;
+        printf("keff = %f   (3D segments %ld, sweep %.1f ms on device)\n", keff, segs, tm.sweep_ms);
+    }
+    double t_total = t_transport + t_exchange + t_renorm + t_update + t_keff;
+    border();
+    center("RESULTS SUMMARY");
+    border();
+    printf("Transport Sweep Time:         %6.2lf sec   (%4.1lf%%)\n", t_transport,
+           100 * t_transport / t_total);
+    printf("Domain Flux Exchange Time:    %6.2lf sec   (%4.1lf%%)\n", t_exchange,
+           100 * t_exchange / t_total);
+    printf("Flux Renormalization Time:    %6.2lf sec   (%4.1lf%%)\n", t_renorm,
+           100 * t_renorm / t_total);
+    printf("Update Source Time:           %6.2lf sec   (%4.1lf%%)\n", t_update,
+           100 * t_update / t_total);
+    printf("K-Effective Calc Time:        %6.2lf sec   (%4.1lf%%)\n", t_keff,
+           100 * t_keff / t_total);
+    printf("Total Time:                   %6.2lf sec\n", t_total);
+    double tpi = t_transport / (double)total_segments * 1.0e9 / (double)in.n_egroups;
+    printf("Time per Intersection:          %.4lf ns\n", tpi);
+    printf("Integrations per Second:        %.4g (device sweep time: %.4g)\n",
+           (double)total_segments * in.n_egroups / t_transport,
+           (double)total_segments * in.n_egroups / (dev_sweep_ms * 1e-3));
+    border();
+    moc_destroy(ctx);
+    moc_problem_free(prob);
+    return 0;
+}

@@ -1,0 +1,83 @@
+"""Shared helpers of the test suite: problem builders and error metrics."""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+
+import simplemoc_amd as sm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+PROBLEM_KEYS = {  # product array name -> reference dump name
+    "az_weight": "az_weight", "seg_offset": "seg_offset", "seg_length": "seg_length",
+    "polar_angle": "polar_angle", "p_weight": "p_weight", "z_height": "z_height0",
+    "material": "material", "vol": "vol", "fine_source": "fine_source0", "sigT": "sigT",
+    "xs": "xs", "scatter": "scatter", "table": "table",
+}
+
+
+def digest(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def make_input(**kw) -> sm.Input:
+    """Reference default input with overrides, derived."""
+    base = dict(x_assemblies=2, y_assemblies=2, cai=2, fai=5, axial_exp=2, radial_ray_sep=0.5,
+                axial_z_sep=0.25, n_azimuthal=4, n_polar_angles=4, n_egroups=8, decompose=1,
+                decomp_assemblies_ax=1, segments_per_track=20, assembly_width=21.42, height=5.0,
+                precision=0.01, n_2D_source_regions_per_assembly=200)
+    base.update(kw)
+    inp = sm.Input(**base)
+    inp.derive()
+    return inp
+
+
+# Named workloads (the shapes of the benchmark configurations)
+CONFIG1 = dict(x_assemblies=2, y_assemblies=2, cai=2, fai=5, axial_exp=2, radial_ray_sep=0.05,
+               axial_z_sep=0.25, n_azimuthal=16, n_polar_angles=4, n_egroups=8, decompose=1,
+               decomp_assemblies_ax=1, segments_per_track=20, assembly_width=21.42, height=20.0,
+               precision=0.01, n_2D_source_regions_per_assembly=5000)
+CONFIG2 = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2, radial_ray_sep=0.05,
+               axial_z_sep=0.25, n_azimuthal=64, n_polar_angles=10, n_egroups=128, decompose=1,
+               decomp_assemblies_ax=20, segments_per_track=40, assembly_width=21.42,
+               height=400.0, precision=0.01, n_2D_source_regions_per_assembly=5000)
+
+
+def oracle_for(problem: sm.Problem, psi: bool = True, **kw):
+    """Oracle holding copies of the problem's arrays.  The region-draw seed
+    defaults to the solver's default (1).  psi=False skips the angular flux
+    (trace-only checks of large shapes)."""
+    from oracle import pyoracle as po
+    arrays = dict(problem.arrays)
+    G, T = problem.dims["G"], problem.dims["ntracks"]
+    if not psi:
+        arrays["f_psi"] = np.zeros(1, np.float32)
+        arrays["b_psi"] = None
+    elif arrays["f_psi"] is None:
+        arrays["f_psi"] = np.zeros(T * G, np.float32)
+    kw.setdefault("seed", 1)
+    return po.Oracle(problem.dims, arrays, **kw)
+
+
+def flux_errors(x, ref, floor_frac=1e-3, G=None):
+    """Normwise and floored per-element relative errors.
+
+    normwise: max|x-ref| / max|ref|
+    floored:  max |x-ref| / max(|ref|, floor_frac * max_g |ref| of the same FSR)
+    """
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    d = np.abs(x - ref)
+    scale = np.abs(ref).max()
+    norm = d.max() / scale if scale > 0 else d.max()
+    if G:
+        fsr_max = np.abs(ref).reshape(-1, G).max(axis=1).repeat(G)
+    else:
+        fsr_max = np.full(ref.shape, np.abs(ref).max())
+    den = np.maximum(np.abs(ref), floor_frac * fsr_max)
+    den[den == 0] = 1.0
+    floored = (d / den).max()
+    return float(norm), float(floored)

@@ -1,0 +1,147 @@
+"""Host-side checks of the native library (no GPU needed).
+
+* the shared library loads and exports every function include/*.h declares;
+* input handling matches the reference (src/init.c, src/io.c);
+* the derived shape matches the oracle's restatement of src/init.c:4-30;
+* the synthetic generator is deterministic and draws the stated
+  distributions;
+* errors surface as status codes with messages (no exit, no crash);
+* creating a device context without a GPU fails loudly.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import simplemoc_amd as sm
+from helpers import CONFIG2, GOLDEN, ROOT, make_input
+from oracle import pyoracle as po
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"//[^\n]*", "", txt)
+        txt = re.sub(r"#[^\n]*", "", txt)
+        for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\(([^;{}()]*)\)\s*;", txt):
+            name = m.group(1)
+            if name not in {"if", "for", "while", "return", "sizeof"}:
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_function():
+    out = subprocess.run(["nm", "-D", "--defined-only", sm.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    decl = declared_functions()
+    assert "moc_transport_sweep" in decl and "transport_sweep" in decl
+    missing = sorted(decl - exported)
+    assert not missing, missing
+
+
+def test_default_and_small_inputs_match_reference():
+    d = sm.Input.default()
+    assert (d.cai, d.fai, d.n_azimuthal, d.n_polar_angles, d.n_egroups, d.decomp_assemblies_ax,
+            d.segments_per_track, d.n_2D_source_regions_per_assembly) == (27, 5, 64, 10, 104, 20, 120, 5000)
+    assert np.float32(d.radial_ray_sep) == np.float32(0.05)
+    s = sm.Input.small()
+    assert (s.cai, s.fai, s.n_azimuthal, s.n_polar_angles, s.decomp_assemblies_ax,
+            s.n_2D_source_regions_per_assembly) == (5, 3, 5, 5, 1, 3000)
+    assert np.float32(s.assembly_width) == np.float32(1.26 * 17)
+
+
+def test_input_file_parsing():
+    inp = sm.Input.from_file(os.path.join(GOLDEN, "tiny_g16.in"))
+    assert (inp.cai, inp.fai, inp.axial_exp, inp.n_azimuthal, inp.n_polar_angles, inp.n_egroups,
+            inp.decomp_assemblies_ax, inp.segments_per_track, inp.n_2D_source_regions_per_assembly,
+            inp.papi_event_set) == (4, 5, 2, 4, 4, 16, 2, 20, 200, 0)
+    assert np.float32(inp.height) == np.float32(10.0)
+    assert np.float32(inp.radial_ray_sep) == np.float32(0.5)
+
+
+@pytest.mark.parametrize("kw", [dict(), CONFIG2,
+                                dict(n_azimuthal=32, n_polar_angles=6, n_egroups=32, height=100.0,
+                                     cai=5, decomp_assemblies_ax=4, radial_ray_sep=0.05)])
+def test_derived_shape_matches_oracle(kw):
+    inp = make_input(**kw)
+    ref = po.derive_shape(n_azimuthal=kw.get("n_azimuthal", 4),
+                          n_polar_angles=inp.n_polar_angles, assembly_width=inp.assembly_width,
+                          radial_ray_sep=inp.radial_ray_sep, height=inp.height,
+                          axial_z_sep=inp.axial_z_sep,
+                          decomp_assemblies_ax=inp.decomp_assemblies_ax,
+                          n_2D_source_regions_per_assembly=inp.n_2D_source_regions_per_assembly,
+                          cai=inp.cai)
+    assert inp.ntracks_2D == ref["ntracks_2D"]
+    assert inp.z_stacked == ref["z_stacked"]
+    assert inp.ntracks == ref["ntracks"]
+    assert inp.n_source_regions_per_node == ref["nsr"]
+    assert np.float32(inp.domain_height) == np.float32(ref["domain_height"])
+
+
+def test_config2_shape():
+    inp = make_input(**CONFIG2)
+    assert (inp.ntracks_2D, inp.z_stacked, inp.ntracks, inp.n_source_regions_per_node) == \
+        (19386, 80, 15508800, 5000)
+
+
+def test_synthetic_generator_deterministic_and_distributed():
+    inp = make_input(n_egroups=16, radial_ray_sep=0.1, segments_per_track=40)
+    a = sm.Problem.generate(inp, seed=4).arrays
+    b = sm.Problem.generate(inp, seed=4).arrays
+    c = sm.Problem.generate(inp, seed=5).arrays
+    for k in ("seg_offset", "seg_length", "p_weight", "sigT", "fine_source", "material"):
+        assert np.array_equal(a[k], b[k]), k
+    assert not np.array_equal(a["seg_length"], c["seg_length"])
+    nseg = np.diff(a["seg_offset"])
+    assert abs(nseg.mean() - 40) < 1.0 and abs(nseg.var() - 40) < 8  # Poisson(40)
+    L = a["seg_length"]
+    assert L.min() >= 0.01 and L.max() <= 0.5 and abs(L.mean() - 0.255) < 0.01
+    s = a["sigT"]
+    assert s.min() >= 0.1 and s.max() <= 2.0
+    assert a["material"][0] == 0 and a["material"].max() < a["xs"].size // (16 * 3)
+
+
+def test_uniform_segment_counts():
+    inp = make_input(radial_ray_sep=0.2)
+    a = sm.Problem.generate(inp, seed=2, seg_dist="uniform", seg_lo=5, seg_hi=500).arrays
+    nseg = np.diff(a["seg_offset"])
+    assert nseg.min() >= 5 and nseg.max() <= 500
+
+
+def test_table_matches_oracle():
+    inp = make_input()
+    p = sm.Problem.generate(inp, seed=1)
+    vals, N, dx, mv = po.build_table(0.01, 10.0)
+    assert p.dims["table_N"] == N == 353
+    assert np.array_equal(p.array("table").view(np.uint32), vals.view(np.uint32))
+    assert np.float32(p.dims["table_dx"]) == np.float32(dx)
+
+
+def test_errors_are_status_codes(tmp_path):
+    with pytest.raises(sm.MOCError, match="cannot open input file"):
+        sm.Input.from_file(tmp_path / "missing.in")
+    bad = tmp_path / "short.in"
+    bad.write_text("1 - x\n2 - y\n")
+    with pytest.raises(sm.MOCError, match="expected 18 values"):
+        sm.Input.from_file(bad)
+    inp = make_input()
+    with pytest.raises(sm.MOCError, match="already derived"):
+        inp.derive()
+    with pytest.raises(sm.MOCError, match="axial expansion"):
+        sm.Input(axial_exp=1).derive()
+
+
+def test_context_without_gpu_fails_loudly():
+    if sm.device_count() > 0:
+        pytest.skip("a GPU is present")
+    inp = make_input()
+    p = sm.Problem.generate(inp, seed=1)
+    with pytest.raises(sm.MOCError, match="no HIP device"):
+        sm.Solver(p)

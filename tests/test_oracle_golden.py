@@ -1,0 +1,70 @@
+"""The oracle against the golden fixtures recorded from the reference.
+
+tests/golden/*.npz were written by tests/golden/make_golden.py from the
+unmodified reference sources (serial -O0 build, fixed srand seeds).  In its
+reference modes (libc rand() region draws, FP32 sequential tally, FP32
+pairwise reductions) the oracle must reproduce every recorded value bit for
+bit: segment counts, k_eff, residuals, flux after the first sweep, final
+flux, source, z heights and angular flux.
+
+The product's MOC_GEN_REFERENCE generator must reproduce the reference's
+generated problem exactly (sha256 of every array).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, PROBLEM_KEYS, digest, oracle_for
+import simplemoc_amd as sm
+from oracle import pyoracle as po
+
+CASES = ["tiny_quad", "tiny_flat", "tiny_g16_decomp2"]
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def problem_of(case, golden_index):
+    meta = golden_index[case]
+    inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
+    inp.derive()
+    return meta, sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_generator_matches_reference_problem(case, golden_index):
+    meta, prob = problem_of(case, golden_index)
+    arrs = prob.arrays
+    for ours, ref in PROBLEM_KEYS.items():
+        assert digest(arrs[ours]) == meta["problem_sha256"][ref], ours
+    d = meta["dims"]
+    assert [prob.dims[k] for k in ("ntracks_2D", "n_polar", "z_stacked", "G", "nsr", "fai")] == d[:6]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_reproduces_reference_bitwise(case, golden_index, libc):
+    meta, prob = problem_of(case, golden_index)
+    g = np.load(os.path.join(GOLDEN, f"{case}.npz"))
+    o = oracle_for(prob, qsr_mode=po.QSR_LIBC, tally=po.TALLY_F32, reduce=po.REDUCE_F32)
+    libc.srand(meta["sweep_seed"])
+    keff = 1.0
+    segs, keffs, ress = [], [], []
+    for it in range(meta["iterations"]):
+        segs.append(o.transport_sweep())
+        if it == 0:
+            assert bits_equal(o["fine_flux"], g["flux_after_sweep0"])
+            assert bits_equal(o["z_height"], g["z_after_sweep0"])
+        o.renormalize_flux()
+        ress.append(o.update_sources(keff))
+        keff = o.compute_keff()
+        keffs.append(keff)
+    assert segs == g["segments"].tolist()
+    assert bits_equal(np.array(keffs, np.float32), g["keff"])
+    assert bits_equal(np.array(ress, np.float32), g["residual"])
+    for k in ("fine_flux", "fine_source", "z_height", "f_psi"):
+        assert bits_equal(o[k], g[k]), k
+    assert bits_equal(np.float32(o.leakage), g["leakage"][0])
