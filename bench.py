@@ -119,10 +119,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tally", default="f64", choices=["f64", "f32"])
     ap.add_argument("--lanes-per-track", type=int, default=0)
+    ap.add_argument("--pieces-per-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only time a plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import simplemoc_amd as sm
+        nbytes = int(args.copy_probe * 2**30)
+        ms = sm.copy_probe(nbytes, iters=20)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": ms,
+                          "gb_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); this build runs "
               "the sweep on a single MI355X and has no multi-GPU path.", file=sys.stderr)
@@ -137,7 +147,8 @@ def main():
     log(f"problem: {inp.ntracks_2D} 2D tracks, {inp.ntracks} 3D tracks, "
         f"{inp.n_source_regions_per_node} regions, G={inp.n_egroups} "
         f"({time.perf_counter() - t_setup:.1f} s)")
-    opts = sm.Options(tally=args.tally, lanes_per_track=args.lanes_per_track)
+    opts = sm.Options(tally=args.tally, lanes_per_track=args.lanes_per_track,
+                      pieces_per_batch=args.pieces_per_batch)
     solver = sm.Solver(prob, opts)
     log(f"device context ready ({time.perf_counter() - t_setup:.1f} s), launch {solver.launch()}")
 

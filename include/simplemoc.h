@@ -21,6 +21,7 @@
 #ifndef SIMPLEMOC_H
 #define SIMPLEMOC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -154,6 +155,7 @@ typedef struct moc_options {
     int keep_b_psi;              /* keep a b_psi array on the device */
     int lanes_per_track;         /* 0 auto; 8, 16, 32 or 64 */
     int device;
+    int pieces_per_batch;        /* attenuation: pieces whose loads issue together (0 auto; 1, 2, 4) */
 } moc_options;
 
 void moc_default_options(moc_options *o);
@@ -210,6 +212,11 @@ int moc_get_launch(moc_ctx *ctx, int *lanes_per_track, int *groups_per_lane);
  * up to cap pieces of (ds, zin, fsr); returns the piece count in *n. */
 int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
                           uint32_t *fsr, long cap, long *n);
+
+/* Diagnostic: mean time of a plain device-to-device copy kernel of `bytes`
+ * over `iters` launches (achievable HBM bandwidth; calibration of
+ * hardware traffic counters against a kernel of known traffic). */
+int moc_copy_probe(int device, size_t bytes, int iters, double *ms_per_copy);
 
 const char *moc_last_error(void);
 int moc_device_count(int *n);

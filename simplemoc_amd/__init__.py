@@ -88,7 +88,7 @@ class _Options(C.Structure):
     _fields_ = [
         ("qsr_mode", C.c_int), ("seed", C.c_uint64), ("tally", C.c_int), ("exp_mode", C.c_int),
         ("boundary", C.c_int), ("keep_b_psi", C.c_int), ("lanes_per_track", C.c_int),
-        ("device", C.c_int),
+        ("device", C.c_int), ("pieces_per_batch", C.c_int),
     ]
 
 
@@ -139,6 +139,7 @@ def lib():
                                        C.c_long, C.POINTER(C.c_long)], C.c_int),
             "moc_last_error": ([], C.c_char_p),
             "moc_device_count": ([C.POINTER(C.c_int)], C.c_int),
+            "moc_copy_probe": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -318,10 +319,11 @@ class Problem:
 
 class Options:
     def __init__(self, qsr="hash", seed=1, tally="f64", exp="table", boundary="reflective",
-                 keep_b_psi=False, lanes_per_track=0, device=0):
+                 keep_b_psi=False, lanes_per_track=0, device=0, pieces_per_batch=0):
         self.qsr, self.seed, self.tally, self.exp = qsr, seed, tally, exp
         self.boundary, self.keep_b_psi = boundary, keep_b_psi
         self.lanes_per_track, self.device = lanes_per_track, device
+        self.pieces_per_batch = pieces_per_batch
 
     def _struct(self):
         o = _Options()
@@ -334,6 +336,7 @@ class Options:
         o.keep_b_psi = int(self.keep_b_psi)
         o.lanes_per_track = self.lanes_per_track
         o.device = self.device
+        o.pieces_per_batch = self.pieces_per_batch
         return o
 
 
@@ -445,6 +448,13 @@ class Solver:
                                            C.byref(n)))
         m = min(n.value, cap)
         return ds[:m].copy(), zin[:m].copy(), fsr[:m].copy()
+
+
+def copy_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:
+    """Mean milliseconds of one device-to-device copy of nbytes."""
+    ms = C.c_double()
+    _check(lib().moc_copy_probe(device, nbytes, iters, C.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

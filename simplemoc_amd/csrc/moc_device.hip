@@ -144,147 +144,338 @@ __global__ void k_prep(const float *__restrict__ src, const float *__restrict__ 
 }
 
 // ---------------------------------------------------------------------
-// (2) axial ray trace, one thread per z-stack (src/solver.c:366-527),
-// including the live [begin, end) update inside the k loop and the z reset
-// on every piece of a track's last 2D segment.
-template <bool WRITE>
-__global__ void k_trace(TraceArgs a)
+// (2) axial ray trace (src/solver.c:366-527).  One workgroup per
+// (2D track, polar angle) z-stack; thread `tid` owns the stacked tracks
+// k = tid*KPT .. tid*KPT+KPT-1 with their z heights in registers.
+//
+// The reference walks k serially inside each 2D segment n and changes the
+// loop bounds while it runs: going up, every exit does end-- before the
+// next k is tested; going down, begin++ only affects later segments.  Here
+// every active track first traces segment n tentatively (pieces, exit,
+// final z).  Going up, track k is processed iff k < end - E(k), with E(k)
+// the exits of the lower tracks; that test fails monotonically, so the
+// processed tracks are a prefix [begin, kstop) found with one block scan
+// and one block minimum.  Processed tracks then commit (and, in the write
+// pass, re-trace to emit records); the others keep their z.  The result
+// equals the serial loop, including the quirk of retiring the wrong track
+// when several exit in one segment.
+struct SegResult {
+    int pieces;
+    int exited;
+    float zfin;
+};
+
+struct EmitCtx {
+    float4 *rec;
+    long long rec_cap;
+    long long base;             // record index of the track's next piece
+    const int32_t *stream;      // explicit region draws (or null)
+    long long serial;           // serial index of the first piece (stream mode)
+    uint64_t tkey;
+    long long nsr;
+    int fai;
+    int quad;
+    float dz_att;
+    unsigned *err;
+};
+
+// One (track, 2D segment): the while loop of src/solver.c:409-525.
+template <bool EMIT>
+__device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k, bool pos,
+                                                   bool last_seg, long long n, const TraceArgs &a,
+                                                   double c, EmitCtx *ec)
 {
+    float sr = s_full;
+    int curr = pos ? (int)(zk / a.dzf) : (int)ceil((double)(zk / a.dzf));
+    bool complete = false;
+    int p = 0;
+    int exited = 0;
+    while (!complete) {
+        bool reset = false;
+        float ds;
+        float zn = (float)((double)zk + (double)sr * c);
+        int ni = pos ? (int)(zn / a.dzf) : (int)ceil((double)(zn / a.dzf));
+        if (ni == curr) {
+            complete = true;
+            ds = sr;
+        } else {
+            curr += pos ? 1 : -1;
+            zn = (float)(a.fine_dz * (double)(float)curr);
+            ds = (float)((double)(zn - zk) / c);
+            sr -= ds;
+            if (sr <= 0.f)
+                complete = true;
+            if (zn <= 0.f || (double)zn >= a.node_dz) {
+                complete = true;
+                exited = 1;
+                reset = true;
+            }
+        }
+        if (EMIT) {
+            uint32_t qsr;
+            if (ec->stream) {
+                int32_t v = ec->stream[ec->serial + p];
+                if (v < 0 || (long long)v >= ec->nsr) {
+                    atomicOr(ec->err, ERR_QSR_RANGE);
+                    v = 0;
+                }
+                qsr = (uint32_t)v;
+            } else {
+                qsr = piece_region(ec->tkey, n, p, ec->nsr);
+            }
+            // cell of the piece start, as attenuate_fluxes computes it (:38-45)
+            const int cell = (int)(zk / ec->dz_att);
+            float zin = zk - ec->dz_att * ((float)cell + 0.5f);
+            int fine = cell % ec->fai;
+            if (ec->quad) {
+                if (fine == 0)
+                    zin -= ec->dz_att;
+                else if (fine == ec->fai - 1)
+                    zin += ec->dz_att;
+            }
+            if (fine < 0) {
+                atomicOr(ec->err, ERR_FINE_RANGE);
+                fine = 0;
+            }
+            const long long q = ec->base + p;
+            if (q < ec->rec_cap)
+                ec->rec[q] = make_float4(ds, zin, __uint_as_float(qsr * (uint32_t)ec->fai + (uint32_t)fine), 0.f);
+            else
+                atomicOr(ec->err, ERR_RECORD_CAP);
+        }
+        p++;
+        if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
+            atomicOr(a.err, ERR_TRACE_RUNAWAY);
+            complete = true;
+        }
+        if (last_seg || reset)
+            zk = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
+        else
+            zk = zn;
+    }
+    SegResult r;
+    r.pieces = p;
+    r.exited = exited;
+    r.zfin = zk;
+    return r;
+}
+
+// block-wide helpers (blockDim.x a multiple of 64, at most 1024)
+__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_w[w] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int q = 0; q < nw; q++) {
+        int t = s_w[q];
+        pre += (q < w) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__device__ __forceinline__ int block_min(int v, int *s_w)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    if (lane == 0)
+        s_w[w] = v;
+    __syncthreads();
+    int m = s_w[0];
+    for (int q = 1; q < nw; q++)
+        m = min(m, s_w[q]);
+    __syncthreads();
+    return m;
+}
+
+template <bool WRITE, int KPT>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a)
+{
+    __shared__ int s_w[16];
     const long long nst = a.nt2 * a.np;
-    for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < nst;
-         s += (long long)gridDim.x * blockDim.x) {
+    const int zs = a.zs;
+    const int k0 = threadIdx.x * KPT;
+    for (long long s = blockIdx.x; s < nst; s += gridDim.x) {
         const long long i = s / a.np;
         const int j = (int)(s - i * a.np);
-        const int zs = a.zs;
         const long long t0 = s * zs;
         const bool pos = j < a.np / 2;
         const double c = a.cosd[j];
         const double sn = a.sind[j];
-        float *z = WRITE ? a.z + t0 : a.zwork + t0;
-        for (int k = 0; k < zs; k++) {
-            if (!WRITE)
-                a.zwork[t0 + k] = a.z[t0 + k];
-            a.cnt[t0 + k] = 0;
+        float z[KPT];
+        int cnt[KPT];
+        long long base[KPT];
+        uint64_t tkey[KPT];
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = k0 + r;
+            z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
+            cnt[r] = 0;
+            base[r] = (WRITE && k < zs) ? a.off[t0 + k] : 0;
+            tkey[r] = (WRITE && !a.qsr_stream) ? track_key(a.sweep_key, t0 + k) : 0;
         }
         int begin = 0, end = zs;
+        long long serial = (WRITE && a.qsr_stream) ? a.stack_off[s] : 0;
         const long long off = a.seg_off[i];
         const long long nseg = a.seg_off[i + 1] - off;
-        long long total = 0;
-        const long long sbase = (WRITE && a.qsr_stream) ? a.stack_off[s] : 0;
         for (long long n = 0; n < nseg; n++) {
             const float s_full = (float)((double)a.seg_len[off + n] / sn);
-            for (int k = begin; k < end; k++) {
-                const long long t = t0 + k;
-                float zk = z[k];
-                float sr = s_full;
-                int curr = pos ? (int)(zk / a.dzf) : (int)ceil((double)(zk / a.dzf));
-                bool complete = false;
-                int p = 0;
-                int cur = a.cnt[t];
-                uint64_t tkey = 0;
-                if (WRITE && !a.qsr_stream)
-                    tkey = track_key(a.sweep_key, t);
-                while (!complete) {
-                    bool reset = false;
-                    float ds;
-                    float zn = (float)((double)zk + (double)sr * c);
-                    int ni = pos ? (int)(zn / a.dzf) : (int)ceil((double)(zn / a.dzf));
-                    if (ni == curr) {
-                        complete = true;
-                        ds = sr;
-                    } else {
-                        curr += pos ? 1 : -1;
-                        zn = (float)(a.fine_dz * (double)(float)curr);
-                        ds = (float)((double)(zn - zk) / c);
-                        sr -= ds;
-                        if (sr <= 0.f)
-                            complete = true;
-                        if (zn <= 0.f || (double)zn >= a.node_dz) {
-                            complete = true;
-                            if (pos)
-                                end--;
-                            else
-                                begin++;
-                            reset = true;
-                        }
-                    }
-                    if (WRITE) {
-                        uint32_t qsr;
-                        if (a.qsr_stream) {
-                            int32_t v = a.qsr_stream[sbase + total];
-                            if (v < 0 || (long long)v >= a.nsr) {
-                                atomicOr(a.err, ERR_QSR_RANGE);
-                                v = 0;
-                            }
-                            qsr = (uint32_t)v;
-                        } else {
-                            qsr = piece_region(tkey, n, p, a.nsr);
-                        }
-                        // cell of the piece start, as attenuate_fluxes computes it
-                        const int cell = (int)(zk / a.dz_att);
-                        float zin = zk - a.dz_att * ((float)cell + 0.5f);
-                        int fine = cell % a.fai;
-                        if (a.quad) {
-                            if (fine == 0)
-                                zin -= a.dz_att;
-                            else if (fine == a.fai - 1)
-                                zin += a.dz_att;
-                        }
-                        if (fine < 0) {
-                            atomicOr(a.err, ERR_FINE_RANGE);
-                            fine = 0;
-                        }
-                        const long long q = a.off[t] + cur;
-                        if (q < a.rec_cap) {
-                            a.rec_ds[q] = ds;
-                            a.rec_zin[q] = zin;
-                            a.rec_fsr[q] = qsr * (uint32_t)a.fai + (uint32_t)fine;
-                        } else {
-                            atomicOr(a.err, ERR_RECORD_CAP);
-                        }
-                    }
-                    cur++;
-                    total++;
-                    p++;
-                    if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
-                        atomicOr(a.err, ERR_TRACE_RUNAWAY);
-                        complete = true;
-                    }
-                    if (n == nseg - 1 || reset)
-                        zk = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
-                    else
-                        zk = zn;
+            const bool last = (n == nseg - 1);
+            SegResult res[KPT];
+            bool act[KPT];
+            int nx = 0;
+#pragma unroll
+            for (int r = 0; r < KPT; r++) {
+                const int k = k0 + r;
+                act[r] = (k >= begin && k < end);
+                if (act[r]) {
+                    res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
+                } else {
+                    res[r].pieces = 0;
+                    res[r].exited = 0;
+                    res[r].zfin = z[r];
                 }
-                z[k] = zk;
-                a.cnt[t] = cur;
+                nx += res[r].exited;
+            }
+            // which active tracks the serial loop reaches in this segment
+            bool proc[KPT];
+            int ex_total;
+            int ex_pre = block_excl_scan(nx, s_w, &ex_total);
+            if (pos) {
+                int fail = INT_MAX;
+                int e = ex_pre;
+#pragma unroll
+                for (int r = 0; r < KPT; r++) {
+                    const int k = k0 + r;
+                    if (act[r] && k >= end - e && fail == INT_MAX)
+                        fail = k;
+                    e += res[r].exited;
+                }
+                const int kstop = (ex_total == 0) ? INT_MAX : block_min(fail, s_w);
+                int nproc_exits = 0;
+#pragma unroll
+                for (int r = 0; r < KPT; r++) {
+                    proc[r] = act[r] && (k0 + r) < kstop;
+                    nproc_exits += proc[r] ? res[r].exited : 0;
+                }
+                if (kstop != INT_MAX) {
+                    int tot;
+                    (void)block_excl_scan(nproc_exits, s_w, &tot);
+                    end -= tot;
+                } else {
+                    end -= ex_total;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < KPT; r++)
+                    proc[r] = act[r];
+                begin += ex_total;
+            }
+            // serial index of each track's first piece (region stream)
+            long long ser = 0;
+            if (WRITE && a.qsr_stream) {
+                int np_loc = 0;
+#pragma unroll
+                for (int r = 0; r < KPT; r++)
+                    np_loc += proc[r] ? res[r].pieces : 0;
+                int ptot;
+                int ppre = block_excl_scan(np_loc, s_w, &ptot);
+                ser = serial + ppre;
+                serial += ptot;
+            }
+#pragma unroll
+            for (int r = 0; r < KPT; r++) {
+                if (!proc[r])
+                    continue;
+                if (WRITE) {
+                    EmitCtx ec;
+                    ec.rec = a.rec;
+                    ec.rec_cap = a.rec_cap;
+                    ec.base = base[r] + cnt[r];
+                    ec.stream = a.qsr_stream;
+                    ec.serial = ser;
+                    ec.tkey = tkey[r];
+                    ec.nsr = a.nsr;
+                    ec.fai = a.fai;
+                    ec.quad = a.quad;
+                    ec.dz_att = a.dz_att;
+                    ec.err = a.err;
+                    (void)trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
+                    ser += res[r].pieces;
+                }
+                cnt[r] += res[r].pieces;
+                z[r] = res[r].zfin;
             }
         }
-        if (!WRITE && a.stack_cnt)
-            a.stack_cnt[s] = total;
+        long long stack_total = 0;
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = k0 + r;
+            if (k < zs) {
+                if (WRITE)
+                    a.z[t0 + k] = z[r];
+                else
+                    a.cnt[t0 + k] = cnt[r];
+            }
+            stack_total += cnt[r];
+        }
+        if (!WRITE && a.stack_cnt) {
+            int tot;
+            (void)block_excl_scan((int)stack_total, s_w, &tot);
+            if (threadIdx.x == 0)
+                a.stack_cnt[s] = tot;
+        }
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------
 // (3) attenuation (src/solver.c:14-280 quadratic, :1040-1138 flat)
-__device__ static inline float exp_term(const float2 *tab, float tau, const AttArgs &a)
+// 1 - exp(-tau): the reference table (src/solver.c:1441-1464) or -expm1f.
+// The index clamp is branch-free: v <= 0 (and NaN) -> cell 0, which is the
+// reference's truncation for -1 < v < 0; v >= N-1 -> cell N-1 (never
+// reached below maxVal).
+template <int EXPM>
+__device__ __forceinline__ float exp_term(const float2 *tab, float tau, float dx, float hdx,
+                                          float maxv, int N)
 {
-    if (a.exp_mode != MOC_EXP_TABLE)
+    if (EXPM != MOC_EXP_TABLE)
         return -expm1f(-tau);
-    if (tau > a.maxv)
-        return 1.0f;
-    float v = tau / a.dx + a.hdx;
-    int iv;
-    if (!(v > -1.0f))
-        iv = 0;
-    else if (v >= (float)a.N)
-        iv = a.N - 1;
-    else
-        iv = (int)v;
-    float2 sl = tab[iv];
-    return sl.x * tau + sl.y;
+    const float v = tau / dx + hdx;
+    const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
+    const float2 sl = tab[iv];
+    const float F = sl.x * tau + sl.y;
+    return tau > maxv ? 1.0f : F;
 }
 
-template <int LPT, int GPL, bool QUAD, bool T64>
+// broadcast lane `src` of x to the lanes of its track (LPT lanes per track)
+template <int LPT>
+__device__ __forceinline__ float bcast(float x, int src_in_track, int lane_base)
+{
+    if (LPT == 64)
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src_in_track));
+    return __shfl(x, lane_base + src_in_track, 64);
+}
+
+// One track per LPT lanes; lane l owns groups g = l + r*LPT (r < GPL).
+// Records are read LPT at a time (one coalesced load) and broadcast; the
+// coefficients of PB pieces are requested together so their latency, and
+// the drain of earlier atomics the compiler waits for, is paid once per PB
+// pieces.
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
 __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
 {
     extern __shared__ float2 s_tab[];
@@ -294,10 +485,20 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
 
     constexpr int TPB = 256 / LPT;   // tracks per block
     const int l = threadIdx.x % LPT;
+    const int lane_base = (threadIdx.x % 64) - l;
+    const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
+                                 : (int)(threadIdx.x / LPT);
     const int G = a.G;
+    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
+    const int N = a.N;
+    const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
+    int gcl[GPL];
+#pragma unroll
+    for (int r = 0; r < GPL; r++)
+        gcl[r] = min(l + r * LPT, G - 1);   // clamped group for unconditional loads
     const long long nblk = (a.T + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const long long t = b * TPB + threadIdx.x / LPT;
+        const long long t = b * TPB + slot;
         if (t >= a.T)
             continue;
         const int j = (int)((t / a.zs) % a.np);
@@ -308,52 +509,71 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
         const float weight = QUAD ? a.pw[t] * azw : a.pw[t] * azw * mu;
         float psi[GPL];
 #pragma unroll
-        for (int r = 0; r < GPL; r++) {
-            const int g = l + r * LPT;
-            psi[r] = (g < G) ? a.psi[t * G + g] : 0.f;
-        }
+        for (int r = 0; r < GPL; r++)
+            // a pending renormalisation (src/solver.c:1224) is applied here
+            psi[r] = a.psi[t * G + gcl[r]] * pscale;
         const long long q0 = a.off[t], q1 = a.off[t + 1];
-        for (long long q = q0; q < q1; ++q) {
-            const float ds = a.rec_ds[q];
-            const float zin = a.rec_zin[q];
-            const uint32_t fsr = a.rec_fsr[q];
-            const float4 *cf = a.coef + (size_t)fsr * G;
+        for (long long qc = q0; qc < q1; qc += LPT) {
+            const int nq = (int)min((long long)LPT, q1 - qc);
+            const float4 mine = a.rec[qc + min(l, nq - 1)];
+            for (int p = 0; p < nq; p += PB) {
+                float ds[PB], zin[PB];
+                uint32_t fsr[PB];
+                float4 cf[PB][GPL];
 #pragma unroll
-            for (int r = 0; r < GPL; r++) {
-                const int g = l + r * LPT;
-                if (g >= G)
-                    continue;
-                const float4 c = cf[g];
-                const float sig = c.w;
-                const float tau = sig * ds;
-                const float F = exp_term(s_tab, tau, a);
-                const float ps = psi[r];
-                float tally;
-                if (QUAD) {
-                    const float q0v = c.x + c.y * zin + c.z * zin * zin;
-                    const float q1v = c.y + 2.f * c.z * zin;
-                    const float q2v = c.z;
-                    const float sig2 = sig * sig;
-                    const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * sig2);
-                    const float fi = (q0v * tau + (sig * ps - q0v) * F) / sig2 + q1v * mu * reuse +
-                                     q2v * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) /
-                                         (3.f * sig2 * sig2);
-                    tally = weight * fi;
-                    const float t1 = q0v * F / sig;
-                    const float t2 = q1v * mu * (tau - F) / sig2;
-                    const float t3 = q2v * mu2 * reuse;
-                    const float t4 = ps * (1.f - F);
-                    psi[r] = t1 + t2 + t3 + t4;
-                } else {
-                    const float qv = c.x / sig;
-                    const float dpsi = (ps - qv) * F;
-                    tally = weight * dpsi;
-                    psi[r] = ps - dpsi;
+                for (int u = 0; u < PB; u++) {
+                    const int pu = min(p + u, nq - 1);
+                    ds[u] = bcast<LPT>(mine.x, pu, lane_base);
+                    zin[u] = bcast<LPT>(mine.y, pu, lane_base);
+                    fsr[u] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
+#pragma unroll
+                    for (int r = 0; r < GPL; r++)
+                        cf[u][r] = a.coef[(size_t)fsr[u] * G + gcl[r]];
                 }
-                if (T64)
-                    atomicAdd(a.acc64 + (size_t)fsr * G + g, (double)tally);
-                else
-                    atomicAdd(a.flux32 + (size_t)fsr * G + g, tally);
+#pragma unroll
+                for (int u = 0; u < PB; u++) {
+                    if (p + u >= nq)
+                        break;
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        const int g = l + r * LPT;
+                        if (g >= G)
+                            continue;
+                        const float4 c = cf[u][r];
+                        const float sig = c.w;
+                        const float tau = sig * ds[u];
+                        const float F = exp_term<EXPM>(s_tab, tau, dx, hdx, maxv, N);
+                        const float ps = psi[r];
+                        float tally;
+                        if (QUAD) {
+                            const float zi = zin[u];
+                            const float q0v = c.x + c.y * zi + c.z * zi * zi;
+                            const float q1v = c.y + 2.f * c.z * zi;
+                            const float q2v = c.z;
+                            const float sig2 = sig * sig;
+                            const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * sig2);
+                            const float fi = (q0v * tau + (sig * ps - q0v) * F) / sig2 +
+                                             q1v * mu * reuse +
+                                             q2v * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) /
+                                                 (3.f * sig2 * sig2);
+                            tally = weight * fi;
+                            const float t1 = q0v * F / sig;
+                            const float t2 = q1v * mu * (tau - F) / sig2;
+                            const float t3 = q2v * mu2 * reuse;
+                            const float t4 = ps * (1.f - F);
+                            psi[r] = t1 + t2 + t3 + t4;
+                        } else {
+                            const float qv = c.x / sig;
+                            const float dpsi = (ps - qv) * F;
+                            tally = weight * dpsi;
+                            psi[r] = ps - dpsi;
+                        }
+                        if (T64)
+                            atomicAdd(a.acc64 + (size_t)fsr[u] * G + g, (double)tally);
+                        else
+                            atomicAdd(a.flux32 + (size_t)fsr[u] * G + g, tally);
+                    }
+                }
             }
         }
 #pragma unroll
@@ -643,6 +863,14 @@ __global__ void k_fill_u32(unsigned *__restrict__ x, long long n, unsigned v)
         x[e] = v;
 }
 
+// plain streaming copy (bandwidth probe and counter calibration)
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
 // ---------------------------------------------------------------------
 // context
 struct moc_ctx {
@@ -654,11 +882,13 @@ struct moc_ctx {
     long long nt2 = 0, T = 0, nsr = 0, nfsr = 0, nval = 0, n_mat = 0, n_seg2d = 0;
     int G = 0, np = 0, zs = 0, fai = 0, cai = 0, decomp = 0, quad = 1;
     int lpt = 64, gpl = 1;
+    int att_pb = 4;             // pieces whose coefficients are requested together
     // geometry
     long long *d_segoff = nullptr;
     float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
     double *d_cosd = nullptr, *d_sind = nullptr;
-    float *d_pw = nullptr, *d_z = nullptr, *d_zwork = nullptr, *d_psi = nullptr, *d_bpsi = nullptr;
+    float *d_pw = nullptr, *d_z = nullptr, *d_psi = nullptr, *d_bpsi = nullptr;
+    bool psi_pending = false;   // psi still owes the last renormalisation factor
     // sources
     float *d_src = nullptr, *d_flux = nullptr, *d_sig = nullptr, *d_vol = nullptr;
     int *d_mat = nullptr;
@@ -671,8 +901,7 @@ struct moc_ctx {
     // trace
     int *d_cnt = nullptr;
     long long *d_off = nullptr, *d_stack_cnt = nullptr, *d_stack_off = nullptr;
-    float *d_rec_ds = nullptr, *d_rec_zin = nullptr;
-    uint32_t *d_rec_fsr = nullptr;
+    float4 *d_rec = nullptr;
     long long rec_cap = 0;
     int32_t *d_stream = nullptr;
     long long stream_cap = 0;
@@ -723,10 +952,10 @@ static int dalloc(Tp **p, size_t count)
 static void free_all(moc_ctx *c)
 {
     void *ptrs[] = {c->d_segoff, c->d_seglen, c->d_az, c->d_mu, c->d_cosd, c->d_sind, c->d_pw,
-                    c->d_z, c->d_zwork, c->d_psi, c->d_bpsi, c->d_src, c->d_flux, c->d_sig,
+                    c->d_z, c->d_psi, c->d_bpsi, c->d_src, c->d_flux, c->d_sig,
                     c->d_vol, c->d_mat, c->d_xs, c->d_scT, c->d_coef, c->d_acc, c->d_table,
-                    c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec_ds,
-                    c->d_rec_zin, c->d_rec_fsr, c->d_stream, c->d_scan_tmp, c->d_partial,
+                    c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec,
+                    c->d_stream, c->d_scan_tmp, c->d_partial,
                     c->d_gres, c->d_rres, c->d_scal, c->d_err};
     for (void *p : ptrs)
         if (p)
@@ -749,6 +978,7 @@ extern "C" void moc_default_options(moc_options *o)
     o->keep_b_psi = 0;
     o->lanes_per_track = 0;
     o->device = 0;
+    o->pieces_per_batch = 0;
 }
 
 extern "C" int moc_device_count(int *n)
@@ -788,6 +1018,10 @@ static int choose_launch(moc_ctx *c)
                              G, gpl, lpt);
     c->lpt = lpt;
     c->gpl = gpl_p2;
+    const int pb = c->opt.pieces_per_batch;
+    if (pb != 0 && pb != 1 && pb != 2 && pb != 4)
+        return moc_set_error(MOC_ERR_INVALID, "pieces_per_batch must be 0, 1, 2 or 4");
+    c->att_pb = pb == 0 ? 4 : pb;
     return MOC_OK;
 }
 
@@ -868,7 +1102,7 @@ extern "C" int moc_create(const moc_problem *p, const moc_options *o_in, moc_ctx
     if ((rc = dalloc(&c->d_segoff, nt2 + 1)) || (rc = dalloc(&c->d_seglen, c->n_seg2d)) ||
         (rc = dalloc(&c->d_az, nt2)) || (rc = dalloc(&c->d_mu, c->np)) ||
         (rc = dalloc(&c->d_cosd, c->np)) || (rc = dalloc(&c->d_sind, c->np)) ||
-        (rc = dalloc(&c->d_pw, T)) || (rc = dalloc(&c->d_z, T)) || (rc = dalloc(&c->d_zwork, T)) ||
+        (rc = dalloc(&c->d_pw, T)) || (rc = dalloc(&c->d_z, T)) ||
         (rc = dalloc(&c->d_psi, (size_t)T * G)) || (rc = dalloc(&c->d_src, c->nval)) ||
         (rc = dalloc(&c->d_flux, c->nval)) || (rc = dalloc(&c->d_sig, nsr * G)) ||
         (rc = dalloc(&c->d_vol, nsr)) || (rc = dalloc(&c->d_mat, nsr)) ||
@@ -994,14 +1228,11 @@ static TraceArgs trace_args(moc_ctx *c)
     a.cosd = c->d_cosd;
     a.sind = c->d_sind;
     a.z = c->d_z;
-    a.zwork = c->d_zwork;
     a.cnt = c->d_cnt;
     a.stack_cnt = c->d_stack_cnt;
     a.off = c->d_off;
     a.stack_off = c->d_stack_off;
-    a.rec_ds = c->d_rec_ds;
-    a.rec_zin = c->d_rec_zin;
-    a.rec_fsr = c->d_rec_fsr;
+    a.rec = c->d_rec;
     a.rec_cap = c->rec_cap;
     a.qsr_stream = nullptr;
     a.err = c->d_err;
@@ -1021,21 +1252,40 @@ static TraceArgs trace_args(moc_ctx *c)
     return a;
 }
 
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM>
+static void launch_att_pb(moc_ctx *c, const AttArgs &a, int grid, size_t lds)
+{
+    switch (c->att_pb) {
+    case 1:
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 1>), dim3(grid), dim3(256), lds, c->stream, a);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 2>), dim3(grid), dim3(256), lds, c->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 4>), dim3(grid), dim3(256), lds, c->stream, a);
+        break;
+    }
+}
+
 template <int LPT, int GPL>
 static void launch_att_lpt(moc_ctx *c, const AttArgs &a, int grid, size_t lds)
 {
     const bool t64 = c->opt.tally == MOC_TALLY_F64;
+    const bool tab = c->opt.exp_mode == MOC_EXP_TABLE;
+#define MOC_ATT(Q, T, E) launch_att_pb<LPT, GPL, Q, T, E>(c, a, grid, lds)
     if (c->quad) {
         if (t64)
-            hipLaunchKernelGGL((k_attenuate<LPT, GPL, true, true>), dim3(grid), dim3(256), lds, c->stream, a);
+            tab ? MOC_ATT(true, true, MOC_EXP_TABLE) : MOC_ATT(true, true, MOC_EXP_INTRINSIC);
         else
-            hipLaunchKernelGGL((k_attenuate<LPT, GPL, true, false>), dim3(grid), dim3(256), lds, c->stream, a);
+            tab ? MOC_ATT(true, false, MOC_EXP_TABLE) : MOC_ATT(true, false, MOC_EXP_INTRINSIC);
     } else {
         if (t64)
-            hipLaunchKernelGGL((k_attenuate<LPT, GPL, false, true>), dim3(grid), dim3(256), lds, c->stream, a);
+            tab ? MOC_ATT(false, true, MOC_EXP_TABLE) : MOC_ATT(false, true, MOC_EXP_INTRINSIC);
         else
-            hipLaunchKernelGGL((k_attenuate<LPT, GPL, false, false>), dim3(grid), dim3(256), lds, c->stream, a);
+            tab ? MOC_ATT(false, false, MOC_EXP_TABLE) : MOC_ATT(false, false, MOC_EXP_INTRINSIC);
     }
+#undef MOC_ATT
 }
 
 static void launch_attenuate(moc_ctx *c, const AttArgs &a)
@@ -1058,23 +1308,56 @@ static void launch_attenuate(moc_ctx *c, const AttArgs &a)
     }
 }
 
+// one workgroup per z-stack; KPT stacked tracks per thread
+static int trace_kpt(int zs) { return zs <= 256 ? 1 : zs <= 512 ? 2 : zs <= 1024 ? 4 : 8; }
+
+static void launch_trace(moc_ctx *c, const TraceArgs &ta, bool write)
+{
+    const long long nst = c->nt2 * c->np;
+    const int kpt = trace_kpt(c->zs);
+    int nt = (c->zs + kpt - 1) / kpt;
+    nt = ((nt + 63) / 64) * 64;
+    const int grid = (int)(nst < (1 << 20) ? nst : (1 << 20));
+#define MOC_TRACE_LAUNCH(K)                                                                     \
+    do {                                                                                       \
+        if (write)                                                                             \
+            hipLaunchKernelGGL((k_trace<true, K>), dim3(grid), dim3(nt), 0, c->stream, ta);    \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_trace<false, K>), dim3(grid), dim3(nt), 0, c->stream, ta);   \
+    } while (0)
+    switch (kpt) {
+    case 1: MOC_TRACE_LAUNCH(1); break;
+    case 2: MOC_TRACE_LAUNCH(2); break;
+    case 4: MOC_TRACE_LAUNCH(4); break;
+    default: MOC_TRACE_LAUNCH(8); break;
+    }
+#undef MOC_TRACE_LAUNCH
+}
+
+// apply a pending renormalisation to psi in memory (before anything reads it)
+static int materialize_psi(moc_ctx *c)
+{
+    if (!c->psi_pending)
+        return MOC_OK;
+    const long long npsi = c->T * c->G;
+    hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_psi,
+                       npsi, c->d_scal);
+    HIP_TRY(hipGetLastError());
+    c->psi_pending = false;
+    return MOC_OK;
+}
+
 static int ensure_records(moc_ctx *c, long long S)
 {
     if (S <= c->rec_cap)
         return MOC_OK;
     long long cap = S + S / 8 + 1024;
-    if (c->d_rec_ds) {
-        HIP_TRY(hipFree(c->d_rec_ds));
-        HIP_TRY(hipFree(c->d_rec_zin));
-        HIP_TRY(hipFree(c->d_rec_fsr));
-        c->d_rec_ds = nullptr;
-        c->d_rec_zin = nullptr;
-        c->d_rec_fsr = nullptr;
+    if (c->d_rec) {
+        HIP_TRY(hipFree(c->d_rec));
+        c->d_rec = nullptr;
         c->rec_cap = 0;
     }
-    TRY(dalloc(&c->d_rec_ds, cap));
-    TRY(dalloc(&c->d_rec_zin, cap));
-    TRY(dalloc(&c->d_rec_fsr, cap));
+    TRY(dalloc(&c->d_rec, cap));
     c->rec_cap = cap;
     return MOC_OK;
 }
@@ -1094,8 +1377,7 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     // (2) count pass
     TraceArgs ta = trace_args(c);
-    const int tgrid = launch_grid(nst, 64);
-    hipLaunchKernelGGL(k_trace<false>, dim3(tgrid), dim3(64), 0, c->stream, ta);
+    launch_trace(c, ta, false);
     HIP_TRY(hipGetLastError());
     // (3) offsets
     size_t tb = c->scan_tmp_bytes;
@@ -1105,9 +1387,7 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     TRY(ensure_records(c, S));
-    ta.rec_ds = c->d_rec_ds;
-    ta.rec_zin = c->d_rec_zin;
-    ta.rec_fsr = c->d_rec_fsr;
+    ta.rec = c->d_rec;
     ta.rec_cap = c->rec_cap;
     if (c->opt.qsr_mode == MOC_QSR_LIBC) {
         // region draws from the host's libc rand() in the reference's serial
@@ -1132,18 +1412,17 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     // (4) write pass
-    hipLaunchKernelGGL(k_trace<true>, dim3(tgrid), dim3(64), 0, c->stream, ta);
+    launch_trace(c, ta, true);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     // (5) attenuation
     AttArgs aa;
     aa.off = c->d_off;
-    aa.rec_ds = c->d_rec_ds;
-    aa.rec_zin = c->d_rec_zin;
-    aa.rec_fsr = c->d_rec_fsr;
+    aa.rec = c->d_rec;
     aa.coef = c->d_coef;
     aa.table = c->d_table;
     aa.psi = c->d_psi;
+    aa.psi_scale = c->psi_pending ? c->d_scal + SC_NF : nullptr;
     aa.acc64 = c->d_acc;
     aa.flux32 = c->d_flux;
     aa.pw = c->d_pw;
@@ -1157,9 +1436,9 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     aa.dx = c->table_dx;
     aa.hdx = 0.5f * c->table_dx;
     aa.maxv = c->table_maxv;
-    aa.exp_mode = c->opt.exp_mode;
     launch_attenuate(c, aa);
     HIP_TRY(hipGetLastError());
+    c->psi_pending = false;
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     // (6) flux = (float)accumulator
     if (c->opt.tally == MOC_TALLY_F64) {
@@ -1219,6 +1498,7 @@ extern "C" int moc_boundary_exchange(moc_ctx *c)
         return MOC_OK;
     }
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    TRY(materialize_psi(c));
     long long nt = vacuum_tracks(c);
     if (nt > c->T)
         nt = c->T;
@@ -1258,12 +1538,13 @@ extern "C" int moc_renormalize_flux(moc_ctx *c)
     hipLaunchKernelGGL(k_scale_flux, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
                        c->d_flux, c->d_vol, c->d_scal, c->nval, c->G, c->fai);
     const long long npsi = c->T * c->G;
-    hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_psi,
-                       npsi, c->d_scal);
+    // psi owes one factor at a time: settle an older one first
+    TRY(materialize_psi(c));
     if (c->d_bpsi)
         hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream,
                            c->d_bpsi, npsi, c->d_scal);
     HIP_TRY(hipGetLastError());
+    c->psi_pending = true;   // applied when the next sweep loads psi
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[1]));
     float ms;
@@ -1331,6 +1612,8 @@ extern "C" int moc_download(moc_ctx *c, moc_problem *dst, unsigned what)
         dst->in.n_source_regions_per_node != c->nsr || dst->in.fai != c->fai)
         return moc_set_error(MOC_ERR_INVALID, "moc_download: problem shape differs from the context");
     HIP_TRY(hipSetDevice(c->device));
+    if (what & MOC_DL_PSI)
+        TRY(materialize_psi(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if ((what & MOC_DL_FLUX) && dst->fine_flux)
         HIP_TRY(hipMemcpy(dst->fine_flux, c->d_flux, c->nval * 4, hipMemcpyDeviceToHost));
@@ -1371,8 +1654,10 @@ extern "C" int moc_upload_state(moc_ctx *c, const moc_problem *src, unsigned wha
         HIP_TRY(hipMemcpy(c->d_flux, src->fine_flux, c->nval * 4, hipMemcpyHostToDevice));
     if ((what & MOC_DL_SOURCE) && src->fine_source)
         HIP_TRY(hipMemcpy(c->d_src, src->fine_source, c->nval * 4, hipMemcpyHostToDevice));
-    if ((what & MOC_DL_PSI) && src->f_psi)
+    if ((what & MOC_DL_PSI) && src->f_psi) {
         HIP_TRY(hipMemcpy(c->d_psi, src->f_psi, (size_t)c->T * c->G * 4, hipMemcpyHostToDevice));
+        c->psi_pending = false;
+    }
     if ((what & MOC_DL_Z) && src->z_height)
         HIP_TRY(hipMemcpy(c->d_z, src->z_height, c->T * 4, hipMemcpyHostToDevice));
     if ((what & MOC_DL_B_PSI) && src->b_psi && c->d_bpsi)
@@ -1415,12 +1700,57 @@ extern "C" int moc_get_track_records(moc_ctx *c, long track, float *ds, float *z
         *n = (long)cnt;
     long long m = cnt < cap ? cnt : cap;
     if (m > 0) {
-        if (ds)
-            HIP_TRY(hipMemcpy(ds, c->d_rec_ds + o[0], m * 4, hipMemcpyDeviceToHost));
-        if (zin)
-            HIP_TRY(hipMemcpy(zin, c->d_rec_zin + o[0], m * 4, hipMemcpyDeviceToHost));
-        if (fsr)
-            HIP_TRY(hipMemcpy(fsr, c->d_rec_fsr + o[0], m * 4, hipMemcpyDeviceToHost));
+        std::vector<float4> r((size_t)m);
+        HIP_TRY(hipMemcpy(r.data(), c->d_rec + o[0], m * sizeof(float4), hipMemcpyDeviceToHost));
+        for (long long q = 0; q < m; q++) {
+            if (ds)
+                ds[q] = r[q].x;
+            if (zin)
+                zin[q] = r[q].y;
+            if (fsr) {
+                float f = r[q].z;
+                memcpy(&fsr[q], &f, 4);
+            }
+        }
     }
+    return MOC_OK;
+}
+
+// Diagnostic: copies `bytes` (rounded down to 16 B) between two device
+// buffers `iters` times with k_copy and reports the mean kernel time.
+extern "C" int moc_copy_probe(int device, size_t bytes, int iters, double *ms_per_copy)
+{
+    if (bytes < 16 || iters < 1)
+        return moc_set_error(MOC_ERR_INVALID, "moc_copy_probe: bytes >= 16 and iters >= 1 required");
+    HIP_TRY(hipSetDevice(device));
+    const long long n = (long long)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    TRY(dalloc(&a, n));
+    if (dalloc(&b, n) != MOC_OK) {
+        (void)hipFree(a);
+        return MOC_ERR_NOMEM;
+    }
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemsetAsync(a, 0, n * 16, st));
+    const int grid = launch_grid(n, 256, 1 << 20);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);   // warm-up
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_copy)
+        *ms_per_copy = ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return MOC_OK;
 }

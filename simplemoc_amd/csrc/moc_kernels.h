@@ -30,14 +30,11 @@ struct TraceArgs {
     const double *cosd;         // [np] cos(polar angle) in double
     const double *sind;         // [np]
     float *z;                   // [T] track z state (updated by the write pass)
-    float *zwork;               // [T] scratch copy for the count pass
-    int *cnt;                   // [T + 1] pieces per track (count) / cursor (write)
+    int *cnt;                   // [T + 1] pieces per track (count pass)
     long long *stack_cnt;       // [nstacks] pieces per stack (count pass, may be null)
     const long long *off;       // [T + 1] record offsets (write pass)
     const long long *stack_off; // [nstacks] serial offsets of stacks (stream mode)
-    float *rec_ds;
-    float *rec_zin;
-    uint32_t *rec_fsr;
+    float4 *rec;                // piece records {ds, zin, fsr bits, 0}
     long long rec_cap;
     const int32_t *qsr_stream;  // explicit region draws in serial order (or null)
     unsigned *err;
@@ -58,12 +55,11 @@ struct TraceArgs {
 // Attenuation: lanes of a track own groups g = lane + r*LPT.
 struct AttArgs {
     const long long *off;
-    const float *rec_ds;
-    const float *rec_zin;
-    const uint32_t *rec_fsr;
+    const float4 *rec;          // piece records {ds, zin, fsr bits, 0}
     const float4 *coef;         // [nfsr][G] {c0, c1, c2, sigma} or {src, 0, 0, sigma}
     const float *table;         // [2N]
     float *psi;                 // [T][G]
+    const float *psi_scale;     // pending renormalisation factor (or null)
     double *acc64;              // [nfsr][G] FP64 tally (or null)
     float *flux32;              // [nfsr][G] FP32 tally target (or null)
     const float *pw;            // [T]
@@ -77,7 +73,6 @@ struct AttArgs {
     float dx;
     float hdx;                  // 0.5f * dx
     float maxv;
-    int exp_mode;
 };
 
 }  // namespace moc
