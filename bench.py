@@ -43,7 +43,8 @@ WORKLOAD = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2, ra
                 height=400.0, precision=0.01, n_2D_source_regions_per_assembly=5000)
 SEED = 2
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "attenuate_traffic_plain_sweep_full.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "attenuate_traffic_plain_sweep_full_{tally}.json")
+METRIC = "3D segment\u00d7group integrations/s, plain vs batched; adjoint/forward time"
 
 
 def log(msg):
@@ -117,7 +118,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--tally", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--tally", default="f32", choices=["f64", "f32"])
+    ap.add_argument("--single-mode", action="store_true",
+                    help="skip the short run of the other tally mode")
     ap.add_argument("--lanes-per-track", type=int, default=0)
     ap.add_argument("--pieces-per-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -147,53 +150,68 @@ def main():
     log(f"problem: {inp.ntracks_2D} 2D tracks, {inp.ntracks} 3D tracks, "
         f"{inp.n_source_regions_per_node} regions, G={inp.n_egroups} "
         f"({time.perf_counter() - t_setup:.1f} s)")
-    opts = sm.Options(tally=args.tally, lanes_per_track=args.lanes_per_track,
-                      pieces_per_batch=args.pieces_per_batch)
-    solver = sm.Solver(prob, opts)
-    log(f"device context ready ({time.perf_counter() - t_setup:.1f} s), launch {solver.launch()}")
-
-    keff = 1.0
-
-    def step(keff):
-        segs = solver.transport_sweep()
-        t = solver.timing()
-        solver.boundary_exchange()
-        solver.renormalize_flux()
-        solver.update_sources(keff)
-        return segs, solver.compute_keff(), t
-
-    for w in range(args.warmup):
-        segs, keff, t = step(keff)
-        log(f"warmup {w}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms")
-    total_int = 0
-    att_ms, att_bytes, sweep_ms = [], [], []
     G, T = inp.n_egroups, inp.ntracks
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        segs, keff, t = step(keff)
-        total_int += segs * G
-        att_ms.append(t["attenuate_ms"])
-        sweep_ms.append(t["sweep_ms"])
-        att_bytes.append(24.0 * segs * G + 8.0 * T * G)
-        log(f"step {k}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms "
-            f"(prep {t['prep_ms']:.1f}, trace {t['trace_ms']:.1f}, attenuate "
-            f"{t['attenuate_ms']:.1f}, finalize {t['finalize_ms']:.1f})")
-    elapsed = time.perf_counter() - t0
-    value = total_int / elapsed
-    att_avg_ms = float(np.mean(att_ms))
-    achieved = float(np.mean(att_bytes)) / (att_avg_ms * 1e-3) / 1e9
+
+    def run_mode(tally, steps, warmup):
+        """Power iterations in one tally mode; returns the measured figures."""
+        opts = sm.Options(tally=tally, lanes_per_track=args.lanes_per_track,
+                          pieces_per_batch=args.pieces_per_batch)
+        with sm.Solver(prob, opts) as solver:
+            log(f"[{tally}] device context ready, launch {solver.launch()}")
+            keff = 1.0
+
+            def step(keff):
+                segs = solver.transport_sweep()
+                t = solver.timing()
+                solver.boundary_exchange()
+                solver.renormalize_flux()
+                solver.update_sources(keff)
+                return segs, solver.compute_keff(), t
+
+            for w in range(warmup):
+                segs, keff, t = step(keff)
+                log(f"[{tally}] warmup {w}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms")
+            total_int = 0
+            att_ms, att_bytes, sweep_ms = [], [], []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                segs, keff, t = step(keff)
+                total_int += segs * G
+                att_ms.append(t["attenuate_ms"])
+                sweep_ms.append(t["sweep_ms"])
+                att_bytes.append(24.0 * segs * G + 8.0 * T * G)
+                log(f"[{tally}] step {k}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms "
+                    f"(prep {t['prep_ms']:.1f}, trace {t['trace_ms']:.1f}, attenuate "
+                    f"{t['attenuate_ms']:.1f}, finalize {t['finalize_ms']:.1f})")
+            elapsed = time.perf_counter() - t0
+        att_avg_ms = float(np.mean(att_ms))
+        return dict(total_int=total_int, elapsed=elapsed, value=total_int / elapsed,
+                    att_ms=att_avg_ms, sweep_ms=float(np.mean(sweep_ms)),
+                    sweep_rate=total_int / (sum(sweep_ms) * 1e-3), keff=keff,
+                    achieved=float(np.mean(att_bytes)) / (att_avg_ms * 1e-3) / 1e9,
+                    pieces=int(total_int // (G * steps)))
+
+    main_run = run_mode(args.tally, args.steps, args.warmup)
+    other = None
+    if not args.single_mode:
+        alt = "f64" if args.tally == "f32" else "f32"
+        r = run_mode(alt, max(1, min(2, args.steps)), 1)
+        other = {"tally": alt, "integrations_per_s": r["value"],
+                 "sweep_only_integrations_per_s": r["sweep_rate"], "attenuate_ms": r["att_ms"],
+                 "roofline_frac": r["achieved"] / HBM_PEAK_GBS}
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILE.format(tally=args.tally)
+    if os.path.exists(tfile):
+        with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     line = {
-        "metric": "3D segment x group integrations/s (plain transport sweep)",
-        "value": value,
+        "metric": METRIC,
+        "value": main_run["value"],
         "unit": "integrations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": main_run["elapsed"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -203,18 +221,20 @@ def main():
             "workload": "plain sweep, one domain at full shape: 17x17 assemblies, 400 cm / 20 "
                         "axial domains, 20 coarse x 5 fine axial intervals, 64 azimuthal x 10 "
                         "polar, 128 groups, Poisson(40) segments",
+            "variant": "plain",
             "ntracks_2D": inp.ntracks_2D, "ntracks_3D": inp.ntracks,
             "source_regions": inp.n_source_regions_per_node, "groups": G,
-            "segment_pieces_per_sweep": int(total_int // (G * args.steps)),
-            "tally": args.tally, "sweep_only_integrations_per_s": total_int / (sum(sweep_ms) * 1e-3),
-            "keff": keff,
+            "segment_pieces_per_sweep": main_run["pieces"],
+            "tally": args.tally,
+            "sweep_only_integrations_per_s": main_run["sweep_rate"],
+            "keff": main_run["keff"],
+            "other_tally": other,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_attenuate", "kernel_ms": att_avg_ms},
+        "roofline": {"bound": "hbm", "achieved": main_run["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": main_run["achieved"] / HBM_PEAK_GBS,
+                     "traffic": traffic, "kernel": "k_attenuate", "kernel_ms": main_run["att_ms"]},
         "cpu_baseline": None,
     }
-    solver.close()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
