@@ -213,6 +213,9 @@ int moc_get_launch(moc_ctx *ctx, int *lanes_per_track, int *groups_per_lane);
 int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
                           uint32_t *fsr, long cap, long *n);
 
+/* Forward angular flux of one track (G floats), renormalisation applied. */
+int moc_get_track_psi(moc_ctx *ctx, long track, float *psi);
+
 /* Diagnostic: mean time of a plain device-to-device copy kernel of `bytes`
  * over `iters` launches (achievable HBM bandwidth; calibration of
  * hardware traffic counters against a kernel of known traffic). */

@@ -15,15 +15,21 @@
  * The structs below repeat the layout of the reference's Input, Segment,
  * Track2D, Track, Source, Table, Params and CommGrid
  * (src/SimpleMOC_header.h:28-158) for the reference's default build:
- * OpenMP on (Source carries a lock pointer), MPI and PAPI off.  Define
- * SIMPLEMOC_DROPIN_NO_OPENMP when the reference objects were built with
- * OPENMP=no.  MPI builds are not supported (one domain per process).
+ * OpenMP on (Source carries a lock pointer), MPI and PAPI off.  Source is
+ * the one struct whose layout depends on the build: the library reads its
+ * stride at run time from SIMPLEMOC_DROPIN_OPENMP (unset or 1: OpenMP
+ * build; 0: OPENMP=no); define SIMPLEMOC_DROPIN_NO_OPENMP when compiling
+ * against this header for a serial build.  MPI builds are not supported
+ * (one domain per process).
  *
  * Behaviour (environment variables read on the first call):
  *   SIMPLEMOC_QSR=hash|libc   region draws: counter hash (default) or the
  *                             host's rand() in the reference's serial order
  *   SIMPLEMOC_SEED=<n>        hash seed (default 1)
  *   SIMPLEMOC_DEVICE=<n>      HIP device (default 0)
+ *   SIMPLEMOC_DROPIN_OPENMP=0 the reference objects were built with OPENMP=no
+ *   SIMPLEMOC_DROPIN_SYNC_PSI=0  do not copy the angular flux back after
+ *                             each call (saves a full PCIe transfer of psi)
  * After every call the state the reference would have changed (scalar
  * flux, angular flux, z heights, sources, segments_processed) is copied
  * back into the caller's arrays, so the caller sees the reference's

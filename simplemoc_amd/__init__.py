@@ -137,6 +137,7 @@ def lib():
             "moc_get_launch": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
             "moc_get_track_records": ([C.c_void_p, C.c_long, _fp, _fp, C.POINTER(C.c_uint32),
                                        C.c_long, C.POINTER(C.c_long)], C.c_int),
+            "moc_get_track_psi": ([C.c_void_p, C.c_long, _fp], C.c_int),
             "moc_last_error": ([], C.c_char_p),
             "moc_device_count": ([C.POINTER(C.c_int)], C.c_int),
             "moc_copy_probe": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -436,6 +437,12 @@ class Solver:
         lpt, gpl = C.c_int(), C.c_int()
         _check(lib().moc_get_launch(self._h, C.byref(lpt), C.byref(gpl)))
         return lpt.value, gpl.value
+
+    def track_psi(self, track: int):
+        G = self.problem._p.contents.inp.n_egroups
+        out = np.zeros(G, np.float32)
+        _check(lib().moc_get_track_psi(self._h, track, out.ctypes.data_as(_fp)))
+        return out
 
     def track_records(self, track: int, cap: int = 1 << 16):
         ds = np.zeros(cap, np.float32)

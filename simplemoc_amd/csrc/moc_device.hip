@@ -1754,3 +1754,14 @@ extern "C" int moc_copy_probe(int device, size_t bytes, int iters, double *ms_pe
     (void)hipFree(b);
     return MOC_OK;
 }
+
+extern "C" int moc_get_track_psi(moc_ctx *c, long track, float *psi_out)
+{
+    if (!c || !psi_out || track < 0 || track >= c->T)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_track_psi: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(materialize_psi(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(psi_out, c->d_psi + (size_t)track * c->G, (size_t)c->G * 4, hipMemcpyDeviceToHost));
+    return MOC_OK;
+}

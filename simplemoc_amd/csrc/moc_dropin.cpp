@@ -37,6 +37,32 @@ struct DropIn {
 
 DropIn g_state;
 
+// The reference's Source struct ends with an OpenMP lock pointer only in
+// OpenMP builds (src/SimpleMOC_header.h:117-119); its stride is read from
+// SIMPLEMOC_DROPIN_OPENMP (1 = reference default build, 0 = OPENMP=no).
+struct SourceHead {
+    float **fine_flux;
+    float **fine_source;
+    float vol;
+    float *sigT;
+    float **XS;
+    float **scattering_matrix;
+};
+
+size_t source_stride()
+{
+    const char *v = getenv("SIMPLEMOC_DROPIN_OPENMP");
+    const bool omp = !(v && v[0] == '0');
+    return omp ? sizeof(SourceHead) + sizeof(void *) : sizeof(SourceHead);
+}
+
+inline const SourceHead &src_at(const moc_ref_Params *P, long r)
+{
+    static const size_t stride = source_stride();
+    return *reinterpret_cast<const SourceHead *>(reinterpret_cast<const char *>(P->sources) +
+                                                 (size_t)r * stride);
+}
+
 [[noreturn]] void die(const char *what)
 {
     fprintf(stderr, "simplemoc drop-in: %s failed: %s\n", what, moc_last_error());
@@ -140,8 +166,8 @@ void bind(const moc_ref_Params *P, const moc_ref_Input *I)
     s.mat.resize(nsr);
     std::vector<long> first;
     for (long r = 0; r < nsr; r++) {
-        auto key = std::make_pair((const void *)P->sources[r].XS,
-                                  (const void *)P->sources[r].scattering_matrix);
+        auto key = std::make_pair((const void *)src_at(P, r).XS,
+                                  (const void *)src_at(P, r).scattering_matrix);
         auto it = mats.find(key);
         if (it == mats.end()) {
             it = mats.emplace(key, (int)first.size()).first;
@@ -153,7 +179,7 @@ void bind(const moc_ref_Params *P, const moc_ref_Input *I)
     s.xs.resize((size_t)nm * G * 3);
     s.scatter.resize((size_t)nm * G * G);
     for (long m = 0; m < nm; m++) {
-        const moc_ref_Source &sr = P->sources[first[m]];
+        const SourceHead &sr = src_at(P, first[m]);
         for (int g = 0; g < G; g++) {
             for (int c = 0; c < 3; c++)
                 s.xs[((size_t)m * G + g) * 3 + c] = sr.XS[g][c];
@@ -165,7 +191,7 @@ void bind(const moc_ref_Params *P, const moc_ref_Input *I)
     s.flux.resize((size_t)nsr * fai * G);
     s.sig.resize((size_t)nsr * G);
     for (long r = 0; r < nsr; r++) {
-        const moc_ref_Source &sr = P->sources[r];
+        const SourceHead &sr = src_at(P, r);
         s.vol[r] = sr.vol;
         memcpy(&s.sig[(size_t)r * G], sr.sigT, G * sizeof(float));
         for (int j = 0; j < fai; j++) {
@@ -236,10 +262,10 @@ void write_back(const moc_ref_Params *P, const moc_ref_Input *I, unsigned what)
         for (long r = 0; r < s.nsr; r++)
             for (int j = 0; j < fai; j++) {
                 if (what & MOC_DL_FLUX)
-                    memcpy(P->sources[r].fine_flux[j], &s.flux[((size_t)r * fai + j) * G],
+                    memcpy(src_at(P, r).fine_flux[j], &s.flux[((size_t)r * fai + j) * G],
                            G * sizeof(float));
                 if (what & MOC_DL_SOURCE)
-                    memcpy(P->sources[r].fine_source[j], &s.src[((size_t)r * fai + j) * G],
+                    memcpy(src_at(P, r).fine_source[j], &s.src[((size_t)r * fai + j) * G],
                            G * sizeof(float));
             }
     }

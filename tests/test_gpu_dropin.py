@@ -1,0 +1,42 @@
+"""The drop-in boundary end to end.
+
+oracle/_ref/ref_main_serial is the reference's own main.c with all of its
+sources (serial -O0, time() pinned so srand is repeatable).
+oracle/_ref/ref_main_dropin is the same main.c and generators with solver.o
+left out and libsimplemoc_hip.so linked in its place: transport_sweep,
+renormalize_flux, update_sources and compute_keff then run on the GPU.
+With the reference's rand() region draws (SIMPLEMOC_QSR=libc) both print
+the same k_eff.  Skipped where the harnesses were not built.
+"""
+import os
+import re
+import subprocess
+
+import pytest
+
+from helpers import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ref_main_serial")
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "ref_main_dropin")
+
+
+def keff_of(cmd, env=None):
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    m = re.findall(r"keff = ([-+0-9.eEnaNinf]+)", out.stdout)
+    assert m, out.stdout[-2000:]
+    return float(m[-1])
+
+
+@pytest.mark.skipif(not (os.path.exists(REF) and os.path.exists(DROPIN)),
+                    reason="reference main harnesses not built")
+@pytest.mark.parametrize("inp", ["tiny.in", "tiny_flat.in", "tiny_g16.in"])
+def test_reference_main_with_gpu_solver(inp):
+    path = os.path.join(GOLDEN, inp)
+    k_ref = keff_of([REF, "-i", path])
+    env = dict(os.environ, SIMPLEMOC_QSR="libc", SIMPLEMOC_DROPIN_OPENMP="0")
+    k_gpu = keff_of([DROPIN, "-i", path], env)
+    print(f"{inp}: reference keff {k_ref:.6f}, drop-in GPU keff {k_gpu:.6f}")
+    assert abs(k_gpu - k_ref) <= 2e-6 + 1e-5 * abs(k_ref)

@@ -149,6 +149,44 @@ def test_matches_reference_golden_run(case, golden_index, libc):
     assert np.all(e_gpu <= REF_TOL + e_64)
 
 
+@pytest.mark.parametrize("case", ["tiny_quad", "tiny_flat", "tiny_g16_decomp2"])
+def test_fp32_tally_as_accurate_as_reference(case, golden_index, libc):
+    """FP32 atomics (the default tally) accumulate in a different order than
+    the reference's serial FP32 sum, so both carry FP32 rounding noise of
+    the same size.  Checked: k_eff and normwise flux within REF_TOL of the
+    reference's run; and the GPU's per-FSR distance to the FP64 build is of
+    the reference's own size: RMS over FSRs at most 3x, maximum at most 10x
+    (the maximum over a few thousand FSRs is a heavy-tailed statistic)."""
+    meta = golden_index[case]
+    g = np.load(os.path.join(GOLDEN, f"{case}.npz"))
+    inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference")
+    G = inp.n_egroups
+    iters = meta["iterations"]
+    with sm.Solver(prob, sm.Options(qsr="libc", tally="f32")) as s:
+        libc.srand(meta["sweep_seed"])
+        out = s.iterate(iters)
+        st = s.state()
+    o64 = oracle_for(prob, qsr_mode=po.QSR_LIBC, tally=po.TALLY_F64, reduce=po.REDUCE_F64)
+    libc.srand(meta["sweep_seed"])
+    o64.iterate(iters)
+    assert [x[0] for x in out] == g["segments"].tolist()
+    for (_, k, _), kr in zip(out, g["keff"]):
+        assert abs(k - kr) <= REF_TOL, (k, kr)
+    norm, _ = flux_errors(st["fine_flux"], g["fine_flux"], G=G)
+    e_gpu = per_fsr_rel(st["fine_flux"], o64["fine_flux"], G)
+    e_ref = per_fsr_rel(g["fine_flux"], o64["fine_flux"], G)
+    rms_gpu = float(np.sqrt(np.mean(e_gpu ** 2)))
+    rms_ref = float(np.sqrt(np.mean(e_ref ** 2)))
+    print(f"{case} (FP32 tally): vs FP32 reference normwise {norm:.2e}; per-FSR distance to the "
+          f"FP64 build: GPU max {e_gpu.max():.2e} rms {rms_gpu:.2e}, reference max "
+          f"{e_ref.max():.2e} rms {rms_ref:.2e}")
+    assert norm <= REF_TOL
+    assert rms_gpu <= 3 * rms_ref + NEAR
+    assert e_gpu.max() <= 10 * e_ref.max() + NEAR
+
+
 def test_trace_records_match_oracle_at_config2_shape():
     """Size-independent check at the full config-2 shape: the device trace of
     sampled tracks (piece lengths, in-cell z, FSR index) equals the oracle's
