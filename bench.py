@@ -43,6 +43,13 @@ WORKLOAD = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2, ra
                 height=400.0, precision=0.01, n_2D_source_regions_per_assembly=5000)
 SEED = 2
 HBM_PEAK_GBS = 8000.0
+# adjoint workload: 4x4 assemblies, 100 cm in 4 axial domains, 5 coarse x 5
+# fine intervals, 32 azimuthal x 6 polar angles, 32 groups
+ADJOINT_WORKLOAD = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=2,
+                        radial_ray_sep=0.05, axial_z_sep=0.25, n_azimuthal=32, n_polar_angles=6,
+                        n_egroups=32, decompose=1, decomp_assemblies_ax=4, segments_per_track=20,
+                        assembly_width=21.42, height=100.0, precision=0.01,
+                        n_2D_source_regions_per_assembly=5000)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "attenuate_traffic_plain_sweep_full_{tally}.json")
 METRIC = "3D segment\u00d7group integrations/s, plain vs batched; adjoint/forward time"
 
@@ -113,6 +120,41 @@ def cpu_baseline(prob, target_s: float, threads: int):
     }
 
 
+def adjoint_leg(sm, reps=2):
+    """Forward sweep vs reverse sweep (gradient of the mean scalar flux over a
+    seeded random 5 % of FSRs w.r.t. sigma_t and c0..c2) on the adjoint
+    workload; device times of each call."""
+    inp = sm.Input(**ADJOINT_WORKLOAD)
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=4)
+    nfsr, G = inp.n_source_regions_per_node * inp.fai, inp.n_egroups
+    rng = np.random.default_rng(4)
+    sel = rng.choice(nfsr, nfsr // 20, replace=False)
+    w = np.zeros((nfsr, G), np.float32)
+    w[sel] = 1.0 / (len(sel) * G)
+    out = {}
+    for tally in ("f32", "f64"):
+        with sm.Solver(prob, sm.Options(tally=tally)) as s:
+            s.adjoint(w)                       # warm-up (allocations)
+            t_adj = []
+            for _ in range(reps):
+                s.adjoint(w)
+                t_adj.append(s.timing()["adjoint_ms"])
+            t_fwd = []
+            for _ in range(reps):
+                segs = s.transport_sweep()
+                t_fwd.append(s.timing()["sweep_ms"])
+        out[tally] = (float(np.mean(t_fwd)), float(np.mean(t_adj)), segs)
+    fwd32, adj, segs = out["f32"]
+    fwd64 = out["f64"][0]
+    log(f"adjoint workload: {inp.ntracks} tracks, {segs} pieces, G={G}: forward {fwd32:.1f} ms "
+        f"(FP64 tally {fwd64:.1f} ms), adjoint {adj:.1f} ms")
+    return {"workload": "4x4 assemblies, 100 cm / 4 axial domains, 5 coarse x 5 fine, 32 azimuthal "
+                        "x 6 polar, 32 groups; J = mean flux over a random 5% of FSRs",
+            "pieces": segs, "forward_ms": fwd32, "forward_fp64_tally_ms": fwd64, "adjoint_ms": adj,
+            "adjoint_over_forward": adj / fwd32, "adjoint_over_forward_fp64_tally": adj / fwd64}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +168,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only time a plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -235,6 +278,11 @@ def main():
                      "traffic": traffic, "kernel": "k_attenuate", "kernel_ms": main_run["att_ms"]},
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        try:
+            line["config"]["adjoint"] = adjoint_leg(sm)
+        except Exception as e:  # reported, never fatal to the plain-sweep measurement
+            log(f"adjoint leg failed: {e}")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:

@@ -200,6 +200,8 @@ typedef struct moc_timing {
     double renorm_ms;
     double update_ms;
     double keff_ms;
+    double adjoint_ms;           /* moc_adjoint: all kernels */
+    double adjoint_kernel_ms;    /* moc_adjoint: the reverse-sweep kernel */
     long segments;               /* 3D segment pieces of the last sweep */
     long sweeps;                 /* sweeps done */
 } moc_timing;
@@ -212,6 +214,24 @@ int moc_get_launch(moc_ctx *ctx, int *lanes_per_track, int *groups_per_lane);
  * up to cap pieces of (ds, zin, fsr); returns the piece count in *n. */
 int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
                           uint32_t *fsr, long cap, long *n);
+
+/* ---- differentiation (no counterpart in the reference) ----
+ * J = sum over the next sweep's pieces and groups of weights[fsr][g] times
+ * the scalar-flux tally.  moc_adjoint returns J and its gradient with
+ * respect to sigma_t [nsr][G] and to the quadratic source coefficients
+ * (c0, c1, c2) [nsr*fai][G][3] of that sweep, by a reverse sweep that
+ * recomputes the attenuation per chunk of pieces (FP32 arithmetic, FP64
+ * accumulation).  The solver state (flux, psi, z, sweep count) is not
+ * changed.  With the table exponential the derivative of F is the cell
+ * slope.  Up to 64 groups; tracks up to 1024 pieces. */
+int moc_adjoint(moc_ctx *ctx, const float *weights, double *J, double *grad_sigma,
+                double *grad_coef);
+/* Use the given coefficients (c0,c1,c2 per FSR and group, sigma_t per
+ * region and group) instead of the fit of fine_source in later sweeps and
+ * adjoints; both NULL returns to the fit. */
+int moc_set_coefficients(moc_ctx *ctx, const float *coef3, const float *sigma);
+/* The coefficients the next sweep uses. */
+int moc_get_coefficients(moc_ctx *ctx, float *coef3, float *sigma);
 
 /* Forward angular flux of one track (G floats), renormalisation applied. */
 int moc_get_track_psi(moc_ctx *ctx, long track, float *psi);

@@ -608,3 +608,253 @@ float orc_compute_keff(const orc_problem *P, const orc_opts *o)
     }
     return (float)(fis_d / (abs_d + (double)*P->leakage));
 }
+
+/* ------------------------------------------------------------------ */
+/* FP64 functional and its reverse-mode gradient (no reference
+ * counterpart: the adjoint of the attenuation recurrence of
+ * src/solver.c:14-280 / :1040-1138).
+ *
+ * J = sum over pieces p and groups g of a[fsr_p][g] * tally_{p,g}, i.e. the
+ * weighted scalar flux one sweep tallies, evaluated in double precision
+ * with the quadratic coefficients c0,c1,c2 per (FSR, group) and sigma_t per
+ * (region, group) as independent inputs.  The trace (piece lengths, cells,
+ * regions) is the sweep's own; z heights are not changed.  exp_mode
+ * selects F = 1 - exp(-tau) (intrinsic) or the table evaluated in double
+ * (piecewise-linear; its derivative is the cell slope). */
+static double expval_d(const orc_problem *P, int exp_mode, double tau, double *dF)
+{
+    if (exp_mode == ORC_EXP_INTRINSIC) {
+        double F = -expm1(-tau);
+        *dF = 1.0 - F;
+        return F;
+    }
+    if (tau > (double)P->table_maxVal) {
+        *dF = 0.0;
+        return 1.0;
+    }
+    float v = (float)tau / P->table_dx + 0.5f * P->table_dx;
+    int iv = !(v > -1.0f) ? 0 : (v >= (float)P->table_N ? P->table_N - 1 : (int)v);
+    if (iv < 0)
+        iv = 0;
+    *dF = (double)P->table[2 * iv];
+    return (double)P->table[2 * iv] * tau + (double)P->table[2 * iv + 1];
+}
+
+static double adjoint_impl(const orc_problem *P, orc_opts *o, const double *coef,
+                           const double *sigma, const double *w, int exp_mode,
+                           double *gsig, double *gcoef)
+{
+    const int G = P->G, fai = P->fai, zs = P->z_stacked;
+    const double dz = (double)(P->height / (float)(P->fai * P->decomp * P->cai));
+    const int quad = P->axial_exp == 2;
+    double J = 0.0;
+    float *z = (float *)malloc((size_t)zs * sizeof(float));
+    piece_sink sink;
+    memset(&sink, 0, sizeof(sink));
+    sink.grow = 1;
+    double *psi = (double *)malloc((size_t)G * sizeof(double));
+    double *lam = (double *)malloc((size_t)G * sizeof(double));
+    double *hist = NULL;
+    size_t hist_cap = 0;
+    for (long i = 0; i < P->ntracks_2D; i++) {
+        const double azw = (double)P->az_weight[i];
+        for (int j = 0; j < P->n_polar; j++) {
+            const double mu = (double)(float)cos((double)P->polar_angle[j]);
+            const long s = (long)i * P->n_polar + j;
+            memcpy(z, P->z_height + s * zs, (size_t)zs * sizeof(float));
+            sink.n = 0;
+            trace_stack(P, o, i, j, z, &sink);
+            for (int k = 0; k < zs; k++) {
+                const long t = s * zs + k;
+                /* pieces of track t, in order */
+                long np = 0;
+                for (long q = 0; q < sink.n; q++)
+                    np += sink.v[q].track == t;
+                if ((size_t)(np + 1) * G > hist_cap) {
+                    hist_cap = (size_t)(np + 1) * G;
+                    hist = (double *)realloc(hist, hist_cap * sizeof(double));
+                }
+                for (int g = 0; g < G; g++)
+                    psi[g] = (double)P->f_psi[t * G + g];
+                const double wt = (double)P->p_weight[t] * azw * (quad ? 1.0 : mu);
+                long m = 0;
+                /* forward: record psi at each piece start, accumulate J */
+                for (long q = 0; q < sink.n; q++) {
+                    const orc_piece *pc = &sink.v[q];
+                    if (pc->track != t)
+                        continue;
+                    const double zh = (double)pc->zh;
+                    const int cell = (int)(pc->zh / (float)dz);
+                    double zin = (double)(pc->zh - (float)dz * ((float)cell + 0.5f));
+                    const int fine = cell % fai;
+                    if (quad) {
+                        if (fine == 0) zin -= dz;
+                        else if (fine == fai - 1) zin += dz;
+                    }
+                    (void)zh;
+                    const long fsr = pc->qsr * fai + fine;
+                    for (int g = 0; g < G; g++) {
+                        hist[m * G + g] = psi[g];
+                        const double sg = sigma[pc->qsr * G + g];
+                        const double tau = sg * (double)pc->ds;
+                        double dF;
+                        const double F = expval_d(P, exp_mode, tau, &dF);
+                        const double a = w[fsr * G + g] * wt;
+                        if (quad) {
+                            const double *c = coef + (fsr * G + g) * 3;
+                            const double q0 = c[0] + c[1] * zin + c[2] * zin * zin;
+                            const double q1 = c[1] + 2.0 * c[2] * zin;
+                            const double q2 = c[2];
+                            const double s2 = sg * sg;
+                            const double reuse = tau * (tau - 2.0) + 2.0 * F / (sg * s2);
+                            const double I = (q0 * tau + (sg * psi[g] - q0) * F) / s2 + q1 * mu * reuse +
+                                             q2 * mu * mu * (tau * (tau * (tau - 3.0) + 6.0) - 6.0 * F) / (3.0 * s2 * s2);
+                            J += a * I;
+                            psi[g] = q0 * F / sg + q1 * mu * (tau - F) / s2 + q2 * mu * mu * reuse +
+                                     psi[g] * (1.0 - F);
+                        } else {
+                            const double qv = coef[(fsr * G + g) * 3] / sg;
+                            const double dpsi = (psi[g] - qv) * F;
+                            J += a * dpsi;
+                            psi[g] -= dpsi;
+                        }
+                    }
+                    m++;
+                }
+                if (!gsig && !gcoef)
+                    continue;
+                /* reverse */
+                for (int g = 0; g < G; g++)
+                    lam[g] = 0.0;
+                long mm = m;
+                for (long q = sink.n - 1; q >= 0; q--) {
+                    const orc_piece *pc = &sink.v[q];
+                    if (pc->track != t)
+                        continue;
+                    mm--;
+                    const int cell = (int)(pc->zh / (float)dz);
+                    double zin = (double)(pc->zh - (float)dz * ((float)cell + 0.5f));
+                    const int fine = cell % fai;
+                    if (quad) {
+                        if (fine == 0) zin -= dz;
+                        else if (fine == fai - 1) zin += dz;
+                    }
+                    const long fsr = pc->qsr * fai + fine;
+                    const double ds = (double)pc->ds;
+                    for (int g = 0; g < G; g++) {
+                        const double ps = hist[mm * G + g];
+                        const double sg = sigma[pc->qsr * G + g];
+                        const double tau = sg * ds;
+                        double dF;
+                        const double F = expval_d(P, exp_mode, tau, &dF);
+                        const double b = w[fsr * G + g] * wt;
+                        const double lp = lam[g];
+                        double g_sig, g_c0, g_c1 = 0.0, g_c2 = 0.0, g_psi;
+                        if (quad) {
+                            const double *c = coef + (fsr * G + g) * 3;
+                            const double q0 = c[0] + c[1] * zin + c[2] * zin * zin;
+                            const double q1 = c[1] + 2.0 * c[2] * zin;
+                            const double q2 = c[2];
+                            const double mu2 = mu * mu;
+                            const double s2 = sg * sg, s3 = s2 * sg, s4 = s2 * s2;
+                            const double reuse = tau * (tau - 2.0) + 2.0 * F / s3;
+                            const double P3 = tau * (tau * (tau - 3.0) + 6.0) - 6.0 * F;
+                            /* d/dq */
+                            const double gq0 = b * (tau - F) / s2 + lp * F / sg;
+                            const double gq1 = b * mu * reuse + lp * mu * (tau - F) / s2;
+                            const double gq2 = b * mu2 * P3 / (3.0 * s4) + lp * mu2 * reuse;
+                            g_c0 = gq0;
+                            g_c1 = gq0 * zin + gq1;
+                            g_c2 = gq0 * zin * zin + 2.0 * gq1 * zin + gq2;
+                            /* d/dsigma through sigma, tau and F */
+                            const double A = (q0 * tau + (sg * ps - q0) * F) / s2;
+                            const double C = q2 * mu2 * P3 / (3.0 * s4);
+                            const double dI_ds = ps * F / s2 - 2.0 * A / sg - 6.0 * q1 * mu * F / s4 - 4.0 * C / sg;
+                            const double dI_dt = q0 / s2 + q1 * mu * (2.0 * tau - 2.0) +
+                                                 q2 * mu2 * (3.0 * tau * tau - 6.0 * tau + 6.0) / (3.0 * s4);
+                            const double dI_dF = (sg * ps - q0) / s2 + 2.0 * q1 * mu / s3 - 2.0 * q2 * mu2 / s4;
+                            const double dP_ds = -q0 * F / s2 - 2.0 * q1 * mu * (tau - F) / s3 - 6.0 * q2 * mu2 * F / s4;
+                            const double dP_dt = q1 * mu / s2 + q2 * mu2 * (2.0 * tau - 2.0);
+                            const double dP_dF = q0 / sg - q1 * mu / s2 + 2.0 * q2 * mu2 / s3 - ps;
+                            g_sig = b * (dI_ds + (dI_dt + dI_dF * dF) * ds) +
+                                    lp * (dP_ds + (dP_dt + dP_dF * dF) * ds);
+                            g_psi = b * F / sg + lp * (1.0 - F);
+                        } else {
+                            const double src = coef[(fsr * G + g) * 3];
+                            const double qv = src / sg;
+                            g_c0 = (lp - b) * F / sg;
+                            g_sig = (b - lp) * (qv * F / sg + (ps - qv) * dF * ds);
+                            g_psi = b * F + lp * (1.0 - F);
+                        }
+                        if (gsig)
+                            gsig[pc->qsr * G + g] += g_sig;
+                        if (gcoef) {
+                            gcoef[(fsr * G + g) * 3 + 0] += g_c0;
+                            gcoef[(fsr * G + g) * 3 + 1] += g_c1;
+                            gcoef[(fsr * G + g) * 3 + 2] += g_c2;
+                        }
+                        lam[g] = g_psi;
+                    }
+                }
+            }
+        }
+    }
+    free(z);
+    free(sink.v);
+    free(psi);
+    free(lam);
+    free(hist);
+    return J;
+}
+
+double orc_functional_f64(const orc_problem *P, orc_opts *o, const double *coef,
+                          const double *sigma, const double *weights, int exp_mode)
+{
+    return adjoint_impl(P, o, coef, sigma, weights, exp_mode, NULL, NULL);
+}
+
+double orc_adjoint_f64(const orc_problem *P, orc_opts *o, const double *coef, const double *sigma,
+                       const double *weights, int exp_mode, double *grad_sigma, double *grad_coef)
+{
+    const long nv = P->nsr * P->fai * P->G;
+    if (grad_sigma)
+        memset(grad_sigma, 0, (size_t)P->nsr * P->G * sizeof(double));
+    if (grad_coef)
+        memset(grad_coef, 0, (size_t)nv * 3 * sizeof(double));
+    return adjoint_impl(P, o, coef, sigma, weights, exp_mode, grad_sigma, grad_coef);
+}
+
+/* The quadratic-fit coefficients of src/solver.c:55-138 per (FSR, group),
+ * computed in float exactly as the sweep does (rows clamped for fai < 3),
+ * returned as c0,c1,c2 triples; flat mode returns (source, 0, 0). */
+void orc_fit_coefficients(const orc_problem *P, float *coef)
+{
+    const int G = P->G, fai = P->fai;
+    const float dz = P->height / (float)(P->fai * P->decomp * P->cai);
+    const float two_dz = 2.f * dz, two_dz2 = 2.f * dz * dz;
+    for (long r = 0; r < P->nsr; r++)
+        for (int f = 0; f < fai; f++)
+            for (int g = 0; g < G; g++) {
+                float *c = coef + ((r * fai + f) * G + g) * 3;
+                const float *src = P->fine_source + r * fai * G;
+                if (P->axial_exp == 0) {
+                    c[0] = src[f * G + g];
+                    c[1] = 0.f;
+                    c[2] = 0.f;
+                    continue;
+                }
+                int r1, r2, r3;
+                if (f == 0) { r1 = 0; r2 = 1; r3 = 2; }
+                else if (f == fai - 1) { r1 = f - 2; r2 = f - 1; r3 = f; }
+                else { r1 = f - 1; r2 = f; r3 = f + 1; }
+                if (r1 < 0) r1 = 0;
+                if (r2 < 0) r2 = 0;
+                if (r1 > fai - 1) r1 = fai - 1;
+                if (r2 > fai - 1) r2 = fai - 1;
+                if (r3 > fai - 1) r3 = fai - 1;
+                const float y1 = src[r1 * G + g], y2 = src[r2 * G + g], y3 = src[r3 * G + g];
+                c[0] = y2;
+                c[1] = (y1 - y3) / two_dz;
+                c[2] = (y1 - 2.f * y2 + y3) / two_dz2;
+            }
+}

@@ -171,6 +171,18 @@ void orc_renormalize_flux(orc_problem *P, const orc_opts *o);
 float orc_update_sources(orc_problem *P, float keff);
 float orc_compute_keff(const orc_problem *P, const orc_opts *o);
 
+/* ---- FP64 functional and adjoint (gradient checks) ----
+ * J = sum_{pieces, groups} weights[fsr][g] * tally of the next sweep,
+ * evaluated in double with coef[fsr][g][3] = (c0, c1, c2) and
+ * sigma[region][g] as inputs.  The problem state is not changed. */
+double orc_functional_f64(const orc_problem *P, orc_opts *o, const double *coef,
+                          const double *sigma, const double *weights, int exp_mode);
+double orc_adjoint_f64(const orc_problem *P, orc_opts *o, const double *coef,
+                       const double *sigma, const double *weights, int exp_mode,
+                       double *grad_sigma, double *grad_coef);
+/* the sweep's quadratic-fit coefficients (float), [fsr][g][3] */
+void orc_fit_coefficients(const orc_problem *P, float *coef);
+
 #ifdef __cplusplus
 }
 #endif

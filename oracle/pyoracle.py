@@ -105,6 +105,13 @@ def lib():
         L.orc_update_sources.restype = C.c_float
         L.orc_compute_keff.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_compute_keff.restype = C.c_float
+        _dp = C.POINTER(C.c_double)
+        L.orc_functional_f64.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts), _dp, _dp, _dp, C.c_int]
+        L.orc_functional_f64.restype = C.c_double
+        L.orc_adjoint_f64.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts), _dp, _dp, _dp,
+                                      C.c_int, _dp, _dp]
+        L.orc_adjoint_f64.restype = C.c_double
+        L.orc_fit_coefficients.argtypes = [C.POINTER(OrcProblem), _fp]
         _lib = L
     return _lib
 
@@ -231,6 +238,33 @@ class Oracle:
             keff = self.compute_keff()
             out.append((segs, keff, res))
         return out
+
+    # -- FP64 functional and adjoint
+    def fit_coefficients(self):
+        """The sweep's quadratic-fit coefficients, shape [nfsr*G, 3] (float32)."""
+        n = self.d["nsr"] * self.d["fai"] * self.d["G"]
+        out = np.zeros(n * 3, np.float32)
+        lib().orc_fit_coefficients(C.byref(self.p), _ptr(out, C.c_float))
+        return out.reshape(n, 3)
+
+    def functional_f64(self, coef, sigma, weights, exp_mode=EXP_INTRINSIC) -> float:
+        coef = np.ascontiguousarray(coef, np.float64)
+        sigma = np.ascontiguousarray(sigma, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        return float(lib().orc_functional_f64(C.byref(self.p), C.byref(self.opts),
+                                              _ptr(coef, C.c_double), _ptr(sigma, C.c_double),
+                                              _ptr(weights, C.c_double), exp_mode))
+
+    def adjoint_f64(self, coef, sigma, weights, exp_mode=EXP_INTRINSIC):
+        coef = np.ascontiguousarray(coef, np.float64)
+        sigma = np.ascontiguousarray(sigma, np.float64)
+        weights = np.ascontiguousarray(weights, np.float64)
+        gs = np.zeros(sigma.size, np.float64)
+        gc = np.zeros(coef.size, np.float64)
+        J = lib().orc_adjoint_f64(C.byref(self.p), C.byref(self.opts), _ptr(coef, C.c_double),
+                                  _ptr(sigma, C.c_double), _ptr(weights, C.c_double), exp_mode,
+                                  _ptr(gs, C.c_double), _ptr(gc, C.c_double))
+        return float(J), gs, gc.reshape(coef.shape)
 
     def trace_stack(self, i: int, j: int, z_state=None):
         """Pieces of stack (i, j) from the given (or current) z heights."""

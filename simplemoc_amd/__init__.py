@@ -97,6 +97,7 @@ class _Timing(C.Structure):
         ("sweep_ms", C.c_double), ("prep_ms", C.c_double), ("trace_ms", C.c_double),
         ("attenuate_ms", C.c_double), ("finalize_ms", C.c_double), ("boundary_ms", C.c_double),
         ("renorm_ms", C.c_double), ("update_ms", C.c_double), ("keff_ms", C.c_double),
+        ("adjoint_ms", C.c_double), ("adjoint_kernel_ms", C.c_double),
         ("segments", C.c_long), ("sweeps", C.c_long),
     ]
 
@@ -138,6 +139,10 @@ def lib():
             "moc_get_track_records": ([C.c_void_p, C.c_long, _fp, _fp, C.POINTER(C.c_uint32),
                                        C.c_long, C.POINTER(C.c_long)], C.c_int),
             "moc_get_track_psi": ([C.c_void_p, C.c_long, _fp], C.c_int),
+            "moc_adjoint": ([C.c_void_p, _fp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)], C.c_int),
+            "moc_set_coefficients": ([C.c_void_p, _fp, _fp], C.c_int),
+            "moc_get_coefficients": ([C.c_void_p, _fp, _fp], C.c_int),
             "moc_last_error": ([], C.c_char_p),
             "moc_device_count": ([C.POINTER(C.c_int)], C.c_int),
             "moc_copy_probe": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
@@ -437,6 +442,40 @@ class Solver:
         lpt, gpl = C.c_int(), C.c_int()
         _check(lib().moc_get_launch(self._h, C.byref(lpt), C.byref(gpl)))
         return lpt.value, gpl.value
+
+    # -- differentiation
+    def adjoint(self, weights):
+        """J and its gradients for the next sweep: (J, dJ/dsigma [nsr*G],
+        dJ/dcoef [nsr*fai*G, 3]).  The solver state is not changed."""
+        I = self.problem._p.contents.inp
+        G, nsr, fai = I.n_egroups, I.n_source_regions_per_node, I.fai
+        w = np.ascontiguousarray(weights, np.float32).ravel()
+        if w.size != nsr * fai * G:
+            raise MOCError("weights must have nsr*fai*G entries")
+        J = C.c_double()
+        gs = np.zeros(nsr * G, np.float64)
+        gc = np.zeros(nsr * fai * G * 3, np.float64)
+        _check(lib().moc_adjoint(self._h, w.ctypes.data_as(_fp), C.byref(J),
+                                 gs.ctypes.data_as(C.POINTER(C.c_double)),
+                                 gc.ctypes.data_as(C.POINTER(C.c_double))))
+        return J.value, gs, gc.reshape(-1, 3)
+
+    def set_coefficients(self, coef3=None, sigma=None):
+        if coef3 is None and sigma is None:
+            _check(lib().moc_set_coefficients(self._h, None, None))
+            return
+        c = np.ascontiguousarray(coef3, np.float32).ravel()
+        sg = np.ascontiguousarray(sigma, np.float32).ravel()
+        self._coef_keep = (c, sg)
+        _check(lib().moc_set_coefficients(self._h, c.ctypes.data_as(_fp), sg.ctypes.data_as(_fp)))
+
+    def coefficients(self):
+        I = self.problem._p.contents.inp
+        G, nsr, fai = I.n_egroups, I.n_source_regions_per_node, I.fai
+        c = np.zeros(nsr * fai * G * 3, np.float32)
+        sg = np.zeros(nsr * G, np.float32)
+        _check(lib().moc_get_coefficients(self._h, c.ctypes.data_as(_fp), sg.ctypes.data_as(_fp)))
+        return c.reshape(-1, 3), sg
 
     def track_psi(self, track: int):
         G = self.problem._p.contents.inp.n_egroups

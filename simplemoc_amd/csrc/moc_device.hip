@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a)
         for (int r = 0; r < KPT; r++) {
             const int k = k0 + r;
             if (k < zs) {
-                if (WRITE)
+                if (WRITE && a.commit)
                     a.z[t0 + k] = z[r];
                 else
                     a.cnt[t0 + k] = cnt[r];
@@ -583,6 +583,254 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
                 a.psi[t * G + g] = psi[r];
         }
     }
+}
+
+// ---------------------------------------------------------------------
+// (4) adjoint of the attenuation recurrence (no reference counterpart).
+//
+// J = sum over the sweep's pieces p and groups g of w[fsr_p][g] * tally_{p,g}.
+// Per track the forward pass keeps psi at every K-th piece (checkpoints in
+// LDS) and accumulates J; the reverse pass walks the chunks from the last:
+// it recomputes the chunk's psi from its checkpoint into LDS, then runs
+//   lambda_p = b F/sigma + lambda_{p+1} (1 - F)        (b = w * weight)
+// and adds dJ/dsigma[region][g] and dJ/d(c0,c1,c2)[fsr][g] with FP64
+// atomics.  Nothing per piece is stored beyond one chunk.
+constexpr int ADJ_CKPT = 32;   // checkpoints per track
+constexpr int ADJ_CHUNK = 32;  // pieces per chunk at most (tracks up to 1024 pieces)
+constexpr int ADJ_WAVES = 2;   // waves per block
+
+struct AdjArgs {
+    const long long *off;
+    const float4 *rec;
+    const float4 *coef;
+    const float *table;
+    const float *psi;
+    const float *psi_scale;
+    const float *w;             // [nfsr][G] functional weights
+    double *grad_sigma;         // [nsr][G]
+    double *grad_coef;          // [nfsr][G][3]
+    double *J;                  // one double
+    unsigned *err;
+    const float *pw;
+    const float *az;
+    const float *mu;
+    long long T;
+    int G;
+    int np;
+    int zs;
+    int fai;
+    int N;
+    float dx;
+    float hdx;
+    float maxv;
+};
+
+template <int EXPM>
+__device__ __forceinline__ float exp_term_d(const float2 *tab, float tau, float dx, float hdx,
+                                            float maxv, int N, float *dF)
+{
+    if (EXPM != MOC_EXP_TABLE) {
+        const float F = -expm1f(-tau);
+        *dF = 1.0f - F;
+        return F;
+    }
+    const float v = tau / dx + hdx;
+    const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
+    const float2 sl = tab[iv];
+    const bool sat = tau > maxv;
+    *dF = sat ? 0.0f : sl.x;
+    return sat ? 1.0f : sl.x * tau + sl.y;
+}
+
+// one piece forward: psi update and (optionally) the tally I
+template <bool QUAD, int EXPM>
+__device__ __forceinline__ float piece_forward(const float2 *tab, float dx, float hdx, float maxv,
+                                               int N, float4 c, float ds, float zin, float mu,
+                                               float mu2, float &ps, float *Iout)
+{
+    const float sig = c.w;
+    const float tau = sig * ds;
+    float dF;
+    const float F = exp_term_d<EXPM>(tab, tau, dx, hdx, maxv, N, &dF);
+    if (QUAD) {
+        const float q0 = c.x + c.y * zin + c.z * zin * zin;
+        const float q1 = c.y + 2.f * c.z * zin;
+        const float q2 = c.z;
+        const float s2 = sig * sig;
+        const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * s2);
+        if (Iout)
+            *Iout = (q0 * tau + (sig * ps - q0) * F) / s2 + q1 * mu * reuse +
+                    q2 * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) / (3.f * s2 * s2);
+        ps = q0 * F / sig + q1 * mu * (tau - F) / s2 + q2 * mu2 * reuse + ps * (1.f - F);
+    } else {
+        const float qv = c.x / sig;
+        const float dpsi = (ps - qv) * F;
+        if (Iout)
+            *Iout = dpsi;
+        ps = ps - dpsi;
+    }
+    return F;
+}
+
+template <int LPT, bool QUAD, int EXPM>
+__global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
+{
+    extern __shared__ float2 s_tab[];
+    __shared__ float s_ckpt[ADJ_WAVES][ADJ_CKPT][64];
+    __shared__ float s_chunk[ADJ_WAVES][ADJ_CHUNK][64];
+    for (int e = threadIdx.x; e < a.N; e += blockDim.x)
+        s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
+    __syncthreads();
+
+    constexpr int TPB = 64 * ADJ_WAVES / LPT;
+    const int wv = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int l = threadIdx.x % LPT;
+    const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
+                                 : (int)(threadIdx.x / LPT);
+    const int G = a.G;
+    const int g = min(l, G - 1);
+    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
+    const int N = a.N;
+    const bool live = l < G;
+    const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
+    double Jacc = 0.0;
+    const long long nblk = (a.T + TPB - 1) / TPB;
+    for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const long long t = b * TPB + slot;
+        const bool has = t < a.T;
+        const long long tt = has ? t : 0;
+        const int j = (int)((tt / a.zs) % a.np);
+        const long long i = tt / ((long long)a.zs * a.np);
+        const float mu = a.mu[j];
+        const float mu2 = mu * mu;
+        const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
+        const long long q0 = has ? a.off[tt] : 0;
+        const int P = has ? (int)(a.off[tt + 1] - q0) : 0;
+        int K = (P + ADJ_CKPT - 1) / ADJ_CKPT;
+        if (K < 1)
+            K = 1;
+        if (K > ADJ_CHUNK) {
+            atomicOr(a.err, 16u);
+            continue;
+        }
+        float ps = a.psi[tt * G + g] * pscale;
+        // forward: checkpoints and J
+        for (int p = 0; p < P; p++) {
+            if (p % K == 0)
+                s_ckpt[wv][p / K][lane] = ps;
+            const float4 r = a.rec[q0 + p];
+            const uint32_t fsr = __float_as_uint(r.z);
+            const float4 c = a.coef[(size_t)fsr * G + g];
+            float I;
+            piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c, r.x, r.y, mu, mu2, ps, &I);
+            if (live)
+                Jacc += (double)(a.w[(size_t)fsr * G + g] * wt) * (double)I;
+        }
+        // reverse, chunk by chunk
+        float lam = 0.f;
+        const int nch = (P + K - 1) / K;
+        for (int ch = nch - 1; ch >= 0; ch--) {
+            const int p0 = ch * K;
+            const int p1 = min(P, p0 + K);
+            float x = s_ckpt[wv][ch][lane];
+            for (int p = p0; p < p1; p++) {
+                s_chunk[wv][p - p0][lane] = x;
+                const float4 r = a.rec[q0 + p];
+                const float4 c = a.coef[(size_t)__float_as_uint(r.z) * G + g];
+                piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c, r.x, r.y, mu, mu2, x, nullptr);
+            }
+            for (int p = p1 - 1; p >= p0; p--) {
+                const float4 r = a.rec[q0 + p];
+                const uint32_t fsr = __float_as_uint(r.z);
+                const float4 c = a.coef[(size_t)fsr * G + g];
+                const float psp = s_chunk[wv][p - p0][lane];
+                const float ds = r.x, zin = r.y;
+                const float sig = c.w;
+                const float tau = sig * ds;
+                float dF;
+                const float F = exp_term_d<EXPM>(s_tab, tau, dx, hdx, maxv, N, &dF);
+                const float bw = a.w[(size_t)fsr * G + g] * wt;
+                float gs, gc0, gc1 = 0.f, gc2 = 0.f, gpsi;
+                if (QUAD) {
+                    const float q0v = c.x + c.y * zin + c.z * zin * zin;
+                    const float q1v = c.y + 2.f * c.z * zin;
+                    const float q2v = c.z;
+                    const float s2 = sig * sig, s3 = s2 * sig, s4 = s2 * s2;
+                    const float reuse = tau * (tau - 2.f) + 2.f * F / s3;
+                    const float P3 = tau * (tau * (tau - 3.f) + 6.f) - 6.f * F;
+                    const float gq0 = bw * (tau - F) / s2 + lam * F / sig;
+                    const float gq1 = bw * mu * reuse + lam * mu * (tau - F) / s2;
+                    const float gq2 = bw * mu2 * P3 / (3.f * s4) + lam * mu2 * reuse;
+                    gc0 = gq0;
+                    gc1 = gq0 * zin + gq1;
+                    gc2 = gq0 * zin * zin + 2.f * gq1 * zin + gq2;
+                    const float A = (q0v * tau + (sig * psp - q0v) * F) / s2;
+                    const float Cc = q2v * mu2 * P3 / (3.f * s4);
+                    const float dI_ds = psp * F / s2 - 2.f * A / sig - 6.f * q1v * mu * F / s4 - 4.f * Cc / sig;
+                    const float dI_dt = q0v / s2 + q1v * mu * (2.f * tau - 2.f) +
+                                        q2v * mu2 * (3.f * tau * tau - 6.f * tau + 6.f) / (3.f * s4);
+                    const float dI_dF = (sig * psp - q0v) / s2 + 2.f * q1v * mu / s3 - 2.f * q2v * mu2 / s4;
+                    const float dP_ds = -q0v * F / s2 - 2.f * q1v * mu * (tau - F) / s3 - 6.f * q2v * mu2 * F / s4;
+                    const float dP_dt = q1v * mu / s2 + q2v * mu2 * (2.f * tau - 2.f);
+                    const float dP_dF = q0v / sig - q1v * mu / s2 + 2.f * q2v * mu2 / s3 - psp;
+                    gs = bw * (dI_ds + (dI_dt + dI_dF * dF) * ds) + lam * (dP_ds + (dP_dt + dP_dF * dF) * ds);
+                    gpsi = bw * F / sig + lam * (1.f - F);
+                } else {
+                    const float qv = c.x / sig;
+                    gc0 = (lam - bw) * F / sig;
+                    gs = (bw - lam) * (qv * F / sig + (psp - qv) * dF * ds);
+                    gpsi = bw * F + lam * (1.f - F);
+                }
+                if (live) {
+                    const long long region = fsr / a.fai;
+                    atomicAdd(a.grad_sigma + region * G + g, (double)gs);
+                    double *gcp = a.grad_coef + ((size_t)fsr * G + g) * 3;
+                    atomicAdd(gcp, (double)gc0);
+                    if (QUAD) {
+                        atomicAdd(gcp + 1, (double)gc1);
+                        atomicAdd(gcp + 2, (double)gc2);
+                    }
+                }
+                lam = gpsi;
+            }
+        }
+    }
+    // J: wave then block sum, one atomic per block
+    for (int o = 32; o > 0; o >>= 1)
+        Jacc += __shfl_down(Jacc, o, 64);
+    __shared__ double s_j[ADJ_WAVES];
+    if (lane == 0)
+        s_j[wv] = Jacc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < ADJ_WAVES; q++)
+            s += s_j[q];
+        atomicAdd(a.J, s);
+    }
+}
+
+__global__ void k_prep_user(const float *__restrict__ uc, const float *__restrict__ us,
+                            float4 *__restrict__ coef, const float *__restrict__ flux,
+                            double *__restrict__ acc, long long nval, int G, int fai)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long fsr = e / G;
+        const int g = (int)(e - fsr * G);
+        const long long r = fsr / fai;
+        coef[e] = make_float4(uc[3 * e], uc[3 * e + 1], uc[3 * e + 2], us[r * G + g]);
+        if (acc)
+            acc[e] = (double)flux[e];
+    }
+}
+
+__global__ void k_fill_d(double *__restrict__ x, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] = 0.0;
 }
 
 __global__ void k_finalize(const double *__restrict__ acc, float *__restrict__ flux, long long n)
@@ -907,6 +1155,11 @@ struct moc_ctx {
     long long stream_cap = 0;
     void *d_scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
+    // caller-supplied coefficients and adjoint buffers
+    bool user_coef = false;
+    float *d_user_c = nullptr, *d_user_s = nullptr;
+    float *d_adj_w = nullptr;
+    double *d_adj_gs = nullptr, *d_adj_gc = nullptr, *d_adj_J = nullptr;
     // reductions
     double *d_partial = nullptr;
     float *d_gres = nullptr, *d_rres = nullptr;
@@ -956,7 +1209,8 @@ static void free_all(moc_ctx *c)
                     c->d_vol, c->d_mat, c->d_xs, c->d_scT, c->d_coef, c->d_acc, c->d_table,
                     c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec,
                     c->d_stream, c->d_scan_tmp, c->d_partial,
-                    c->d_gres, c->d_rres, c->d_scal, c->d_err};
+                    c->d_gres, c->d_rres, c->d_scal, c->d_err, c->d_user_c, c->d_user_s,
+                    c->d_adj_w, c->d_adj_gs, c->d_adj_gc, c->d_adj_J};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1210,11 +1464,12 @@ static int check_err(moc_ctx *c)
     if (err) {
         unsigned zero = 0;
         HIP_TRY(hipMemcpy(c->d_err, &zero, 4, hipMemcpyHostToDevice));
-        return moc_set_error(MOC_ERR_STATE, "device check failed (flags 0x%x: %s%s%s%s)", err,
+        return moc_set_error(MOC_ERR_STATE, "device check failed (flags 0x%x: %s%s%s%s%s)", err,
                              (err & ERR_QSR_RANGE) ? "region draw out of range " : "",
                              (err & ERR_FINE_RANGE) ? "negative fine interval " : "",
                              (err & ERR_RECORD_CAP) ? "record capacity exceeded " : "",
-                             (err & ERR_TRACE_RUNAWAY) ? "axial trace runaway" : "");
+                             (err & ERR_TRACE_RUNAWAY) ? "axial trace runaway " : "",
+                             (err & 16u) ? "track longer than the adjoint's 1024-piece limit" : "");
     }
     return MOC_OK;
 }
@@ -1249,6 +1504,7 @@ static TraceArgs trace_args(moc_ctx *c)
     a.dz_att = c->in.height / (float)(c->fai * c->decomp * c->cai);
     a.sep = c->in.axial_z_sep;
     a.sweep_key = sweep_key(c->opt.seed, c->sweeps);
+    a.commit = 1;
     return a;
 }
 
@@ -1362,24 +1618,29 @@ static int ensure_records(moc_ctx *c, long long S)
     return MOC_OK;
 }
 
-extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+static int prep_coefficients(moc_ctx *c, double *acc)
 {
-    if (!c)
-        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
-    HIP_TRY(hipSetDevice(c->device));
-    const long long nst = c->nt2 * c->np;
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    // (1) coefficients + accumulator
-    hipLaunchKernelGGL(k_prep, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
-                       c->d_src, c->d_sig, c->d_flux, c->d_coef, c->d_acc, c->nval, c->G, c->fai,
-                       c->quad, c->in.height / (float)(c->fai * c->decomp * c->cai));
+    if (c->user_coef)
+        hipLaunchKernelGGL(k_prep_user, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
+                           c->d_user_c, c->d_user_s, c->d_coef, c->d_flux, acc, c->nval, c->G, c->fai);
+    else
+        hipLaunchKernelGGL(k_prep, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
+                           c->d_src, c->d_sig, c->d_flux, c->d_coef, acc, c->nval, c->G, c->fai,
+                           c->quad, c->in.height / (float)(c->fai * c->decomp * c->cai));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    // (2) count pass
+    return MOC_OK;
+}
+
+// Axial trace of the next sweep into the piece records: count pass,
+// offsets, then the write pass.  With commit the final z heights are
+// stored (a sweep); without, the z state is left as it is (the adjoint).
+static int trace_records(moc_ctx *c, bool commit, long long *S_out)
+{
+    const long long nst = c->nt2 * c->np;
     TraceArgs ta = trace_args(c);
+    ta.commit = commit ? 1 : 0;
     launch_trace(c, ta, false);
     HIP_TRY(hipGetLastError());
-    // (3) offsets
     size_t tb = c->scan_tmp_bytes;
     HIP_TRY(rocprim::exclusive_scan(c->d_scan_tmp, tb, c->d_cnt, c->d_off, (long long)0,
                                     (size_t)(c->T + 1), rocprim::plus<long long>(), c->stream));
@@ -1411,9 +1672,24 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
         ta.qsr_stream = c->d_stream;
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    // (4) write pass
     launch_trace(c, ta, true);
     HIP_TRY(hipGetLastError());
+    *S_out = S;
+    return MOC_OK;
+}
+
+extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    // (1) coefficients + accumulator
+    TRY(prep_coefficients(c, c->d_acc));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    // (2)-(4) trace: count, offsets, records; z heights committed
+    long long S = 0;
+    TRY(trace_records(c, true, &S));
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     // (5) attenuation
     AttArgs aa;
@@ -1763,5 +2039,148 @@ extern "C" int moc_get_track_psi(moc_ctx *c, long track, float *psi_out)
     TRY(materialize_psi(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(psi_out, c->d_psi + (size_t)track * c->G, (size_t)c->G * 4, hipMemcpyDeviceToHost));
+    return MOC_OK;
+}
+
+// ---------------------------------------------------------------------
+// caller-supplied coefficients, and the adjoint
+extern "C" int moc_set_coefficients(moc_ctx *c, const float *coef3, const float *sigma)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_set_coefficients: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!coef3 && !sigma) {
+        c->user_coef = false;
+        return MOC_OK;
+    }
+    if (!coef3 || !sigma)
+        return moc_set_error(MOC_ERR_INVALID, "moc_set_coefficients: give both arrays or neither");
+    if (!c->d_user_c) {
+        TRY(dalloc(&c->d_user_c, (size_t)c->nval * 3));
+        TRY(dalloc(&c->d_user_s, (size_t)c->nsr * c->G));
+    }
+    HIP_TRY(hipMemcpy(c->d_user_c, coef3, (size_t)c->nval * 3 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_user_s, sigma, (size_t)c->nsr * c->G * 4, hipMemcpyHostToDevice));
+    c->user_coef = true;
+    return MOC_OK;
+}
+
+extern "C" int moc_get_coefficients(moc_ctx *c, float *coef3, float *sigma)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_coefficients: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(prep_coefficients(c, nullptr));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (coef3) {
+        std::vector<float4> v((size_t)c->nval);
+        HIP_TRY(hipMemcpy(v.data(), c->d_coef, (size_t)c->nval * sizeof(float4), hipMemcpyDeviceToHost));
+        for (long long e = 0; e < c->nval; e++) {
+            coef3[3 * e] = v[e].x;
+            coef3[3 * e + 1] = v[e].y;
+            coef3[3 * e + 2] = v[e].z;
+        }
+    }
+    if (sigma)
+        HIP_TRY(hipMemcpy(sigma, c->user_coef ? c->d_user_s : c->d_sig, (size_t)c->nsr * c->G * 4,
+                          hipMemcpyDeviceToHost));
+    return MOC_OK;
+}
+
+template <int LPT, bool QUAD>
+static void launch_adj_lpt(moc_ctx *c, const AdjArgs &a, int grid, size_t lds)
+{
+    if (c->opt.exp_mode == MOC_EXP_TABLE)
+        hipLaunchKernelGGL((k_adjoint<LPT, QUAD, MOC_EXP_TABLE>), dim3(grid), dim3(64 * ADJ_WAVES), lds,
+                           c->stream, a);
+    else
+        hipLaunchKernelGGL((k_adjoint<LPT, QUAD, MOC_EXP_INTRINSIC>), dim3(grid), dim3(64 * ADJ_WAVES),
+                           lds, c->stream, a);
+}
+
+extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *grad_sigma,
+                           double *grad_coef)
+{
+    if (!c || !weights)
+        return moc_set_error(MOC_ERR_INVALID, "moc_adjoint: null argument");
+    if (c->gpl != 1)
+        return moc_set_error(MOC_ERR_INVALID, "moc_adjoint supports up to 64 energy groups (got %d)",
+                             c->G);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    TRY(prep_coefficients(c, nullptr));
+    long long S = 0;
+    TRY(trace_records(c, false, &S));
+    if (!c->d_adj_w) {
+        TRY(dalloc(&c->d_adj_w, (size_t)c->nval));
+        TRY(dalloc(&c->d_adj_gs, (size_t)c->nsr * c->G));
+        TRY(dalloc(&c->d_adj_gc, (size_t)c->nval * 3));
+        TRY(dalloc(&c->d_adj_J, 1));
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_adj_w, weights, (size_t)c->nval * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
+                       c->d_adj_gs, c->nsr * c->G);
+    hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nval * 3, 256)), dim3(256), 0, c->stream,
+                       c->d_adj_gc, c->nval * 3);
+    hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, c->stream, c->d_adj_J, 1LL);
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    AdjArgs aa;
+    aa.off = c->d_off;
+    aa.rec = c->d_rec;
+    aa.coef = c->d_coef;
+    aa.table = c->d_table;
+    aa.psi = c->d_psi;
+    aa.psi_scale = c->psi_pending ? c->d_scal + SC_NF : nullptr;
+    aa.w = c->d_adj_w;
+    aa.grad_sigma = c->d_adj_gs;
+    aa.grad_coef = c->d_adj_gc;
+    aa.J = c->d_adj_J;
+    aa.err = c->d_err;
+    aa.pw = c->d_pw;
+    aa.az = c->d_az;
+    aa.mu = c->d_mu;
+    aa.T = c->T;
+    aa.G = c->G;
+    aa.np = c->np;
+    aa.zs = c->zs;
+    aa.fai = c->fai;
+    aa.N = c->table_N;
+    aa.dx = c->table_dx;
+    aa.hdx = 0.5f * c->table_dx;
+    aa.maxv = c->table_maxv;
+    const int tpb = 64 * ADJ_WAVES / c->lpt;
+    const long long nblk = (c->T + tpb - 1) / tpb;
+    const int grid = (int)(nblk < 1048576 ? nblk : 1048576);
+    const size_t lds = (size_t)c->table_N * sizeof(float2);
+#define MOC_ADJ(L)                                   \
+    do {                                             \
+        if (c->quad)                                 \
+            launch_adj_lpt<L, true>(c, aa, grid, lds); \
+        else                                         \
+            launch_adj_lpt<L, false>(c, aa, grid, lds); \
+    } while (0)
+    switch (c->lpt) {
+    case 8: MOC_ADJ(8); break;
+    case 16: MOC_ADJ(16); break;
+    case 32: MOC_ADJ(32); break;
+    default: MOC_ADJ(64); break;
+    }
+#undef MOC_ADJ
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    TRY(check_err(c));
+    float ms_all = 0, ms_k = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&ms_k, c->ev[1], c->ev[2]));
+    c->timing.adjoint_ms = ms_all;
+    c->timing.adjoint_kernel_ms = ms_k;
+    if (J)
+        HIP_TRY(hipMemcpy(J, c->d_adj_J, 8, hipMemcpyDeviceToHost));
+    if (grad_sigma)
+        HIP_TRY(hipMemcpy(grad_sigma, c->d_adj_gs, (size_t)c->nsr * c->G * 8, hipMemcpyDeviceToHost));
+    if (grad_coef)
+        HIP_TRY(hipMemcpy(grad_coef, c->d_adj_gc, (size_t)c->nval * 3 * 8, hipMemcpyDeviceToHost));
     return MOC_OK;
 }

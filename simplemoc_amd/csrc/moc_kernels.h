@@ -50,6 +50,7 @@ struct TraceArgs {
     float dz_att;               // height/(fai*decomp*cai) in float (src/solver.c:38)
     float sep;                  // axial_z_sep
     uint64_t sweep_key;         // hash key of (seed, sweep)
+    int commit;                 // write pass stores the final z heights
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.
