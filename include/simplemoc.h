@@ -223,7 +223,7 @@ int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
  * recomputes the attenuation per chunk of pieces (FP32 arithmetic, FP64
  * accumulation).  The solver state (flux, psi, z, sweep count) is not
  * changed.  With the table exponential the derivative of F is the cell
- * slope.  Up to 64 groups; tracks up to 1024 pieces. */
+ * slope.  Up to 64 groups. */
 int moc_adjoint(moc_ctx *ctx, const float *weights, double *J, double *grad_sigma,
                 double *grad_coef);
 /* Use the given coefficients (c0,c1,c2 per FSR and group, sigma_t per

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -589,15 +590,20 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
 // (4) adjoint of the attenuation recurrence (no reference counterpart).
 //
 // J = sum over the sweep's pieces p and groups g of w[fsr_p][g] * tally_{p,g}.
-// Per track the forward pass keeps psi at every K-th piece (checkpoints in
-// LDS) and accumulates J; the reverse pass walks the chunks from the last:
-// it recomputes the chunk's psi from its checkpoint into LDS, then runs
+// One lane per (track, group).  The forward pass runs the recurrence once
+// and writes the psi entering every piece to the lane's tape (a per-slot
+// workspace sized for the longest track, reused by every track the slot
+// processes); the reverse pass walks the pieces from the last, reads psi
+// back from the tape and runs
 //   lambda_p = b F/sigma + lambda_{p+1} (1 - F)        (b = w * weight)
-// and adds dJ/dsigma[region][g] and dJ/d(c0,c1,c2)[fsr][g] with FP64
-// atomics.  Nothing per piece is stored beyond one chunk.
-constexpr int ADJ_CKPT = 32;   // checkpoints per track
-constexpr int ADJ_CHUNK = 32;  // pieces per chunk at most (tracks up to 1024 pieces)
+// adding J and the gradients with respect to (c0, c1, c2) and sigma into
+// acc[fsr][4][G] with FP64 atomics (contributions that are exactly zero are
+// skipped: adding 0.0 changes nothing).  k_adj_finish folds the sigma part
+// over the fine axial intervals of each region.
 constexpr int ADJ_WAVES = 2;   // waves per block
+constexpr int ADJ_PB = 4;      // pieces whose loads issue together
+constexpr long long ADJ_MAX_GRID = 4096;              // resident blocks at most
+constexpr long long ADJ_TAPE_FLOATS = 512LL << 20;    // 2 GiB of tape at most
 
 struct AdjArgs {
     const long long *off;
@@ -607,10 +613,10 @@ struct AdjArgs {
     const float *psi;
     const float *psi_scale;
     const float *w;             // [nfsr][G] functional weights
-    double *grad_sigma;         // [nsr][G]
-    double *grad_coef;          // [nfsr][G][3]
+    double *acc;                // [nfsr][4][G]: d/dc0, d/dc1, d/dc2, d/dsigma per FSR
+    float *tape;                // [slots][max_pieces][G]
+    long long tape_stride;      // max_pieces * G
     double *J;                  // one double
-    unsigned *err;
     const float *pw;
     const float *az;
     const float *mu;
@@ -618,7 +624,6 @@ struct AdjArgs {
     int G;
     int np;
     int zs;
-    int fai;
     int N;
     float dx;
     float hdx;
@@ -642,11 +647,11 @@ __device__ __forceinline__ float exp_term_d(const float2 *tab, float tau, float 
     return sat ? 1.0f : sl.x * tau + sl.y;
 }
 
-// one piece forward: psi update and (optionally) the tally I
+// one piece forward: the psi update only
 template <bool QUAD, int EXPM>
-__device__ __forceinline__ float piece_forward(const float2 *tab, float dx, float hdx, float maxv,
-                                               int N, float4 c, float ds, float zin, float mu,
-                                               float mu2, float &ps, float *Iout)
+__device__ __forceinline__ void piece_forward(const float2 *tab, float dx, float hdx, float maxv,
+                                              int N, float4 c, float ds, float zin, float mu,
+                                              float mu2, float &ps)
 {
     const float sig = c.w;
     const float tau = sig * ds;
@@ -658,26 +663,78 @@ __device__ __forceinline__ float piece_forward(const float2 *tab, float dx, floa
         const float q2 = c.z;
         const float s2 = sig * sig;
         const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * s2);
-        if (Iout)
-            *Iout = (q0 * tau + (sig * ps - q0) * F) / s2 + q1 * mu * reuse +
-                    q2 * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) / (3.f * s2 * s2);
         ps = q0 * F / sig + q1 * mu * (tau - F) / s2 + q2 * mu2 * reuse + ps * (1.f - F);
     } else {
-        const float qv = c.x / sig;
-        const float dpsi = (ps - qv) * F;
-        if (Iout)
-            *Iout = dpsi;
-        ps = ps - dpsi;
+        ps = ps - (ps - c.x / sig) * F;
     }
-    return F;
+}
+
+struct PieceGrad {
+    float I;                    // the piece's tally
+    float gs, gc0, gc1, gc2;    // dJ/dsigma, dJ/d(c0, c1, c2) from this piece
+    float lam;                  // dJ/d(psi entering the piece)
+};
+
+// one piece backward: psp is the psi entering the piece, bw the tally's
+// weight in J and lam the adjoint of the psi leaving it
+template <bool QUAD, int EXPM>
+__device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, float hdx,
+                                                   float maxv, int N, float4 c, float ds, float zin,
+                                                   float mu, float mu2, float psp, float bw,
+                                                   float lam)
+{
+    PieceGrad o;
+    const float sig = c.w;
+    const float tau = sig * ds;
+    float dF;
+    const float F = exp_term_d<EXPM>(tab, tau, dx, hdx, maxv, N, &dF);
+    if (QUAD) {
+        const float q0v = c.x + c.y * zin + c.z * zin * zin;
+        const float q1v = c.y + 2.f * c.z * zin;
+        const float q2v = c.z;
+        const float s2 = sig * sig, s3 = s2 * sig, s4 = s2 * s2;
+        const float reuse = tau * (tau - 2.f) + 2.f * F / s3;
+        const float P3 = tau * (tau * (tau - 3.f) + 6.f) - 6.f * F;
+        const float A = (q0v * tau + (sig * psp - q0v) * F) / s2;
+        const float Cc = q2v * mu2 * P3 / (3.f * s4);
+        o.I = A + q1v * mu * reuse + Cc;
+        const float gq0 = bw * (tau - F) / s2 + lam * F / sig;
+        const float gq1 = bw * mu * reuse + lam * mu * (tau - F) / s2;
+        const float gq2 = bw * mu2 * P3 / (3.f * s4) + lam * mu2 * reuse;
+        o.gc0 = gq0;
+        o.gc1 = gq0 * zin + gq1;
+        o.gc2 = gq0 * zin * zin + 2.f * gq1 * zin + gq2;
+        const float dI_ds = psp * F / s2 - 2.f * A / sig - 6.f * q1v * mu * F / s4 - 4.f * Cc / sig;
+        const float dI_dt = q0v / s2 + q1v * mu * (2.f * tau - 2.f) +
+                            q2v * mu2 * (3.f * tau * tau - 6.f * tau + 6.f) / (3.f * s4);
+        const float dI_dF = (sig * psp - q0v) / s2 + 2.f * q1v * mu / s3 - 2.f * q2v * mu2 / s4;
+        const float dP_ds = -q0v * F / s2 - 2.f * q1v * mu * (tau - F) / s3 - 6.f * q2v * mu2 * F / s4;
+        const float dP_dt = q1v * mu / s2 + q2v * mu2 * (2.f * tau - 2.f);
+        const float dP_dF = q0v / sig - q1v * mu / s2 + 2.f * q2v * mu2 / s3 - psp;
+        o.gs = bw * (dI_ds + (dI_dt + dI_dF * dF) * ds) + lam * (dP_ds + (dP_dt + dP_dF * dF) * ds);
+        o.lam = bw * F / sig + lam * (1.f - F);
+    } else {
+        const float qv = c.x / sig;
+        o.I = (psp - qv) * F;
+        o.gc0 = (lam - bw) * F / sig;
+        o.gc1 = 0.f;
+        o.gc2 = 0.f;
+        o.gs = (bw - lam) * (qv * F / sig + (psp - qv) * dF * ds);
+        o.lam = bw * F + lam * (1.f - F);
+    }
+    return o;
+}
+
+__device__ __forceinline__ void add_nonzero(double *p, float v)
+{
+    if (v != 0.0f)
+        atomicAdd(p, (double)v);
 }
 
 template <int LPT, bool QUAD, int EXPM>
 __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
 {
     extern __shared__ float2 s_tab[];
-    __shared__ float s_ckpt[ADJ_WAVES][ADJ_CKPT][64];
-    __shared__ float s_chunk[ADJ_WAVES][ADJ_CHUNK][64];
     for (int e = threadIdx.x; e < a.N; e += blockDim.x)
         s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
     __syncthreads();
@@ -694,6 +751,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
     const int N = a.N;
     const bool live = l < G;
     const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
+    float *tape = a.tape + ((long long)blockIdx.x * TPB + slot) * a.tape_stride + g;
     double Jacc = 0.0;
     const long long nblk = (a.T + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
@@ -707,92 +765,60 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
         const long long q0 = has ? a.off[tt] : 0;
         const int P = has ? (int)(a.off[tt + 1] - q0) : 0;
-        int K = (P + ADJ_CKPT - 1) / ADJ_CKPT;
-        if (K < 1)
-            K = 1;
-        if (K > ADJ_CHUNK) {
-            atomicOr(a.err, 16u);
-            continue;
-        }
         float ps = a.psi[tt * G + g] * pscale;
-        // forward: checkpoints and J
-        for (int p = 0; p < P; p++) {
-            if (p % K == 0)
-                s_ckpt[wv][p / K][lane] = ps;
-            const float4 r = a.rec[q0 + p];
-            const uint32_t fsr = __float_as_uint(r.z);
-            const float4 c = a.coef[(size_t)fsr * G + g];
-            float I;
-            piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c, r.x, r.y, mu, mu2, ps, &I);
-            if (live)
-                Jacc += (double)(a.w[(size_t)fsr * G + g] * wt) * (double)I;
-        }
-        // reverse, chunk by chunk
-        float lam = 0.f;
-        const int nch = (P + K - 1) / K;
-        for (int ch = nch - 1; ch >= 0; ch--) {
-            const int p0 = ch * K;
-            const int p1 = min(P, p0 + K);
-            float x = s_ckpt[wv][ch][lane];
-            for (int p = p0; p < p1; p++) {
-                s_chunk[wv][p - p0][lane] = x;
-                const float4 r = a.rec[q0 + p];
-                const float4 c = a.coef[(size_t)__float_as_uint(r.z) * G + g];
-                piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c, r.x, r.y, mu, mu2, x, nullptr);
+        // forward: psi entering each piece onto the tape
+        for (int p0 = 0; p0 < P; p0 += ADJ_PB) {
+            float4 r[ADJ_PB], c[ADJ_PB];
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++)
+                r[k] = a.rec[q0 + min(p0 + k, P - 1)];
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++)
+                c[k] = a.coef[(size_t)__float_as_uint(r[k].z) * G + g];
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++) {
+                if (p0 + k < P) {
+                    if (live)
+                        tape[(long long)(p0 + k) * G] = ps;
+                    piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k], r[k].x, r[k].y, mu,
+                                              mu2, ps);
+                }
             }
-            for (int p = p1 - 1; p >= p0; p--) {
-                const float4 r = a.rec[q0 + p];
-                const uint32_t fsr = __float_as_uint(r.z);
-                const float4 c = a.coef[(size_t)fsr * G + g];
-                const float psp = s_chunk[wv][p - p0][lane];
-                const float ds = r.x, zin = r.y;
-                const float sig = c.w;
-                const float tau = sig * ds;
-                float dF;
-                const float F = exp_term_d<EXPM>(s_tab, tau, dx, hdx, maxv, N, &dF);
-                const float bw = a.w[(size_t)fsr * G + g] * wt;
-                float gs, gc0, gc1 = 0.f, gc2 = 0.f, gpsi;
-                if (QUAD) {
-                    const float q0v = c.x + c.y * zin + c.z * zin * zin;
-                    const float q1v = c.y + 2.f * c.z * zin;
-                    const float q2v = c.z;
-                    const float s2 = sig * sig, s3 = s2 * sig, s4 = s2 * s2;
-                    const float reuse = tau * (tau - 2.f) + 2.f * F / s3;
-                    const float P3 = tau * (tau * (tau - 3.f) + 6.f) - 6.f * F;
-                    const float gq0 = bw * (tau - F) / s2 + lam * F / sig;
-                    const float gq1 = bw * mu * reuse + lam * mu * (tau - F) / s2;
-                    const float gq2 = bw * mu2 * P3 / (3.f * s4) + lam * mu2 * reuse;
-                    gc0 = gq0;
-                    gc1 = gq0 * zin + gq1;
-                    gc2 = gq0 * zin * zin + 2.f * gq1 * zin + gq2;
-                    const float A = (q0v * tau + (sig * psp - q0v) * F) / s2;
-                    const float Cc = q2v * mu2 * P3 / (3.f * s4);
-                    const float dI_ds = psp * F / s2 - 2.f * A / sig - 6.f * q1v * mu * F / s4 - 4.f * Cc / sig;
-                    const float dI_dt = q0v / s2 + q1v * mu * (2.f * tau - 2.f) +
-                                        q2v * mu2 * (3.f * tau * tau - 6.f * tau + 6.f) / (3.f * s4);
-                    const float dI_dF = (sig * psp - q0v) / s2 + 2.f * q1v * mu / s3 - 2.f * q2v * mu2 / s4;
-                    const float dP_ds = -q0v * F / s2 - 2.f * q1v * mu * (tau - F) / s3 - 6.f * q2v * mu2 * F / s4;
-                    const float dP_dt = q1v * mu / s2 + q2v * mu2 * (2.f * tau - 2.f);
-                    const float dP_dF = q0v / sig - q1v * mu / s2 + 2.f * q2v * mu2 / s3 - psp;
-                    gs = bw * (dI_ds + (dI_dt + dI_dF * dF) * ds) + lam * (dP_ds + (dP_dt + dP_dF * dF) * ds);
-                    gpsi = bw * F / sig + lam * (1.f - F);
-                } else {
-                    const float qv = c.x / sig;
-                    gc0 = (lam - bw) * F / sig;
-                    gs = (bw - lam) * (qv * F / sig + (psp - qv) * dF * ds);
-                    gpsi = bw * F + lam * (1.f - F);
-                }
-                if (live) {
-                    const long long region = fsr / a.fai;
-                    atomicAdd(a.grad_sigma + region * G + g, (double)gs);
-                    double *gcp = a.grad_coef + ((size_t)fsr * G + g) * 3;
-                    atomicAdd(gcp, (double)gc0);
-                    if (QUAD) {
-                        atomicAdd(gcp + 1, (double)gc1);
-                        atomicAdd(gcp + 2, (double)gc2);
+        }
+        // reverse
+        float lam = 0.f;
+        for (int p1 = P - 1; p1 >= 0; p1 -= ADJ_PB) {
+            float4 r[ADJ_PB], c[ADJ_PB];
+            float wv_[ADJ_PB], psp[ADJ_PB];
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++)
+                r[k] = a.rec[q0 + max(p1 - k, 0)];
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++) {
+                const size_t e = (size_t)__float_as_uint(r[k].z) * G + g;
+                c[k] = a.coef[e];
+                wv_[k] = a.w[e];
+                psp[k] = tape[(long long)max(p1 - k, 0) * G];
+            }
+#pragma unroll
+            for (int k = 0; k < ADJ_PB; k++) {
+                if (p1 - k >= 0) {
+                    const float bw = wv_[k] * wt;
+                    const PieceGrad o = piece_reverse<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k],
+                                                                  r[k].x, r[k].y, mu, mu2, psp[k],
+                                                                  bw, lam);
+                    if (live) {
+                        Jacc += (double)bw * (double)o.I;
+                        double *ap = a.acc + (size_t)__float_as_uint(r[k].z) * 4 * G + g;
+                        add_nonzero(ap, o.gc0);
+                        if (QUAD) {
+                            add_nonzero(ap + G, o.gc1);
+                            add_nonzero(ap + 2 * G, o.gc2);
+                        }
+                        add_nonzero(ap + 3 * G, o.gs);
                     }
+                    lam = o.lam;
                 }
-                lam = gpsi;
             }
         }
     }
@@ -808,6 +834,29 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         for (int q = 0; q < ADJ_WAVES; q++)
             s += s_j[q];
         atomicAdd(a.J, s);
+    }
+}
+
+// acc[fsr][4][G] -> grad_coef[fsr][G][3] and grad_sigma[region][G] (the
+// sigma part summed over the region's fine axial intervals in order)
+__global__ void k_adj_finish(const double *__restrict__ acc, double *__restrict__ gc,
+                             double *__restrict__ gs, long long nsr, int G, int fai)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nsr * G;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long r = e / G;
+        const int g = (int)(e - r * G);
+        double sum = 0.0;
+        for (int jf = 0; jf < fai; jf++) {
+            const long long fsr = r * fai + jf;
+            const double *ap = acc + fsr * 4 * G + g;
+            double *gp = gc + (fsr * G + g) * 3;
+            gp[0] = ap[0];
+            gp[1] = ap[G];
+            gp[2] = ap[2 * G];
+            sum += ap[3 * G];
+        }
+        gs[e] = sum;
     }
 }
 
@@ -1159,7 +1208,12 @@ struct moc_ctx {
     bool user_coef = false;
     float *d_user_c = nullptr, *d_user_s = nullptr;
     float *d_adj_w = nullptr;
-    double *d_adj_gs = nullptr, *d_adj_gc = nullptr, *d_adj_J = nullptr;
+    double *d_adj_gs = nullptr, *d_adj_gc = nullptr, *d_adj_J = nullptr, *d_adj_acc = nullptr;
+    int *d_adj_maxp = nullptr;
+    double *d_adj_rtmp = nullptr;
+    size_t adj_rtmp_bytes = 0;
+    float *d_adj_tape = nullptr;
+    long long adj_tape_cap = 0;
     // reductions
     double *d_partial = nullptr;
     float *d_gres = nullptr, *d_rres = nullptr;
@@ -1210,7 +1264,8 @@ static void free_all(moc_ctx *c)
                     c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec,
                     c->d_stream, c->d_scan_tmp, c->d_partial,
                     c->d_gres, c->d_rres, c->d_scal, c->d_err, c->d_user_c, c->d_user_s,
-                    c->d_adj_w, c->d_adj_gs, c->d_adj_gc, c->d_adj_J};
+                    c->d_adj_w, c->d_adj_gs, c->d_adj_gc, c->d_adj_J, c->d_adj_acc,
+                    c->d_adj_maxp, c->d_adj_rtmp, c->d_adj_tape};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1279,7 +1334,25 @@ static int choose_launch(moc_ctx *c)
     return MOC_OK;
 }
 
+static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx **out);
+
+// The HIP runtime's first device initialisation (stream creation) advances
+// or reseeds libc's rand() generator.  Callers that seeded rand() for the
+// reference's region draws (MOC_QSR_LIBC, or a reference driver relinked
+// against this library) must see their sequence untouched, so the context
+// is created on a private generator state and the caller's is put back.
 extern "C" int moc_create(const moc_problem *p, const moc_options *o_in, moc_ctx **out)
+{
+    static std::mutex guard;
+    std::lock_guard<std::mutex> hold(guard);
+    static char private_state[256];
+    char *callers_state = initstate(1u, private_state, sizeof(private_state));
+    const int rc = create_context(p, o_in, out);
+    setstate(callers_state);
+    return rc;
+}
+
+static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx **out)
 {
     if (!p || !out)
         return moc_set_error(MOC_ERR_INVALID, "moc_create: null argument");
@@ -1464,12 +1537,11 @@ static int check_err(moc_ctx *c)
     if (err) {
         unsigned zero = 0;
         HIP_TRY(hipMemcpy(c->d_err, &zero, 4, hipMemcpyHostToDevice));
-        return moc_set_error(MOC_ERR_STATE, "device check failed (flags 0x%x: %s%s%s%s%s)", err,
+        return moc_set_error(MOC_ERR_STATE, "device check failed (flags 0x%x: %s%s%s%s)", err,
                              (err & ERR_QSR_RANGE) ? "region draw out of range " : "",
                              (err & ERR_FINE_RANGE) ? "negative fine interval " : "",
                              (err & ERR_RECORD_CAP) ? "record capacity exceeded " : "",
-                             (err & ERR_TRACE_RUNAWAY) ? "axial trace runaway " : "",
-                             (err & 16u) ? "track longer than the adjoint's 1024-piece limit" : "");
+                             (err & ERR_TRACE_RUNAWAY) ? "axial trace runaway " : "");
     }
     return MOC_OK;
 }
@@ -2116,13 +2188,43 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         TRY(dalloc(&c->d_adj_w, (size_t)c->nval));
         TRY(dalloc(&c->d_adj_gs, (size_t)c->nsr * c->G));
         TRY(dalloc(&c->d_adj_gc, (size_t)c->nval * 3));
+        TRY(dalloc(&c->d_adj_acc, (size_t)c->nval * 4));
         TRY(dalloc(&c->d_adj_J, 1));
+        TRY(dalloc(&c->d_adj_maxp, 1));
+        size_t rb = 0;
+        HIP_TRY(rocprim::reduce(nullptr, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
+                                rocprim::maximum<int>(), c->stream));
+        TRY(dalloc(&c->d_adj_rtmp, rb / 8 + 1));
+        c->adj_rtmp_bytes = rb;
+    }
+    // the longest track sizes the tape
+    size_t rb = c->adj_rtmp_bytes;
+    HIP_TRY(rocprim::reduce(c->d_adj_rtmp, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
+                            rocprim::maximum<int>(), c->stream));
+    int maxp = 0;
+    HIP_TRY(hipMemcpyAsync(&maxp, c->d_adj_maxp, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (maxp < 1)
+        maxp = 1;
+    const int tpb = 64 * ADJ_WAVES / c->lpt;
+    const long long nblk = (c->T + tpb - 1) / tpb;
+    const long long per_block = (long long)tpb * maxp * c->G;   // tape floats per block
+    long long grid = nblk < ADJ_MAX_GRID ? nblk : ADJ_MAX_GRID;
+    if (grid * per_block > ADJ_TAPE_FLOATS)
+        grid = ADJ_TAPE_FLOATS / per_block;
+    if (grid < 1)
+        grid = 1;
+    if (grid * per_block > c->adj_tape_cap) {
+        if (c->d_adj_tape)
+            HIP_TRY(hipFree(c->d_adj_tape));
+        c->d_adj_tape = nullptr;
+        c->adj_tape_cap = 0;
+        TRY(dalloc(&c->d_adj_tape, (size_t)(grid * per_block)));
+        c->adj_tape_cap = grid * per_block;
     }
     HIP_TRY(hipMemcpyAsync(c->d_adj_w, weights, (size_t)c->nval * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
-                       c->d_adj_gs, c->nsr * c->G);
-    hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nval * 3, 256)), dim3(256), 0, c->stream,
-                       c->d_adj_gc, c->nval * 3);
+    hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nval * 4, 256)), dim3(256), 0, c->stream,
+                       c->d_adj_acc, c->nval * 4);
     hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, c->stream, c->d_adj_J, 1LL);
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     AdjArgs aa;
@@ -2133,10 +2235,10 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     aa.psi = c->d_psi;
     aa.psi_scale = c->psi_pending ? c->d_scal + SC_NF : nullptr;
     aa.w = c->d_adj_w;
-    aa.grad_sigma = c->d_adj_gs;
-    aa.grad_coef = c->d_adj_gc;
+    aa.acc = c->d_adj_acc;
+    aa.tape = c->d_adj_tape;
+    aa.tape_stride = (long long)maxp * c->G;
     aa.J = c->d_adj_J;
-    aa.err = c->d_err;
     aa.pw = c->d_pw;
     aa.az = c->d_az;
     aa.mu = c->d_mu;
@@ -2144,21 +2246,17 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     aa.G = c->G;
     aa.np = c->np;
     aa.zs = c->zs;
-    aa.fai = c->fai;
     aa.N = c->table_N;
     aa.dx = c->table_dx;
     aa.hdx = 0.5f * c->table_dx;
     aa.maxv = c->table_maxv;
-    const int tpb = 64 * ADJ_WAVES / c->lpt;
-    const long long nblk = (c->T + tpb - 1) / tpb;
-    const int grid = (int)(nblk < 1048576 ? nblk : 1048576);
     const size_t lds = (size_t)c->table_N * sizeof(float2);
 #define MOC_ADJ(L)                                   \
     do {                                             \
         if (c->quad)                                 \
-            launch_adj_lpt<L, true>(c, aa, grid, lds); \
+            launch_adj_lpt<L, true>(c, aa, (int)grid, lds); \
         else                                         \
-            launch_adj_lpt<L, false>(c, aa, grid, lds); \
+            launch_adj_lpt<L, false>(c, aa, (int)grid, lds); \
     } while (0)
     switch (c->lpt) {
     case 8: MOC_ADJ(8); break;
@@ -2167,6 +2265,9 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     default: MOC_ADJ(64); break;
     }
 #undef MOC_ADJ
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_adj_finish, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
+                       c->d_adj_acc, c->d_adj_gc, c->d_adj_gs, (long long)c->nsr, c->G, c->fai);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[2]));

@@ -234,15 +234,9 @@ void bind(const moc_ref_Params *P, const moc_ref_Input *I)
     o.device = (int)env_long("SIMPLEMOC_DEVICE", 0);
     o.keep_b_psi = bpsi_nonzero ? 1 : 0;
     s.sync_psi = env_long("SIMPLEMOC_DROPIN_SYNC_PSI", 1) != 0;
-    // The reference draws regions from libc's rand() after seeding it
-    // itself; creating the device context initialises the HIP runtime,
-    // which must not advance or reseed that generator.  The context is
-    // created on a private generator state and the caller's is put back.
-    static char private_state[256];
-    char *callers_state = initstate(1u, private_state, sizeof(private_state));
-    const int rc = moc_create(&p, &o, &s.ctx);
-    setstate(callers_state);
-    if (rc != MOC_OK)
+    // moc_create leaves the caller's rand() sequence (seeded by the
+    // reference's main) untouched
+    if (moc_create(&p, &o, &s.ctx) != MOC_OK)
         die("device context creation");
     // the table pointer belongs to the caller; the context holds its copy
     p.table = nullptr;

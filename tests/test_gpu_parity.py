@@ -149,6 +149,28 @@ def test_matches_reference_golden_run(case, golden_index, libc):
     assert np.all(e_gpu <= REF_TOL + e_64)
 
 
+def test_libc_seed_before_context(golden_index, libc):
+    """The reference's main seeds rand() before anything touches the GPU.
+
+    The HIP runtime's first device initialisation advances libc's
+    generator; context creation must leave the caller's sequence as it was,
+    so seeding before creating the solver gives the same draws (segments,
+    z heights, k_eff) as seeding after it."""
+    meta = golden_index["tiny_quad"]
+    g = np.load(os.path.join(GOLDEN, "tiny_quad.npz"))
+    inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference")
+    libc.srand(meta["sweep_seed"])
+    with sm.Solver(prob, sm.Options(qsr="libc")) as s:
+        out = s.iterate(meta["iterations"])
+        st = s.state()
+    assert [x[0] for x in out] == g["segments"].tolist()
+    np.testing.assert_array_equal(st["z_height"], g["z_height"])
+    for (_, k, _), kr in zip(out, g["keff"]):
+        assert abs(k - kr) <= REF_TOL, (k, kr)
+
+
 @pytest.mark.parametrize("case", ["tiny_quad", "tiny_flat", "tiny_g16_decomp2"])
 def test_fp32_tally_as_accurate_as_reference(case, golden_index, libc):
     """FP32 atomics (the default tally) accumulate in a different order than
