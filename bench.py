@@ -155,6 +155,52 @@ def adjoint_leg(sm, reps=2):
             "adjoint_over_forward": adj / fwd32, "adjoint_over_forward_fp64_tally": adj / fwd64}
 
 
+def batched_leg(sm, tally, iters=20):
+    """64 independent fixed-source problems (groups from {8,32,64,128}, 2D
+    segments per track U(5,500), 1-30 fine intervals, seeded): `iters`
+    iterations as one captured graph of persistent batched launches vs the
+    same problems run one after another through the single-problem solver
+    (`iters` sweeps each).  Integrations/s of both and their ratio."""
+    from simplemoc_amd import batch_problems
+    probs, seeds = batch_problems.problems(n=64, seed=3)
+    b = sm.Batch(probs, seeds, tally=tally)
+    try:
+        tune = b.timing()
+        b.run(iters)                       # captures the graph, first replay
+        b.run(iters)
+        t = b.timing()
+        groups = {}
+        for p, prob in enumerate(probs):
+            groups.setdefault(prob.inp.n_egroups, b.tuning(p))
+    finally:
+        b.close()
+    batched_rate = t["integrations"] / (t["run_ms"] * 1e-3)
+    wall, dev_ms, integ = 0.0, 0.0, 0
+    for p, prob in enumerate(probs):
+        with sm.Solver(prob, sm.Options(seed=seeds[p], tally=tally)) as s:
+            s.transport_sweep()
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                integ += s.transport_sweep() * prob.inp.n_egroups
+                dev_ms += s.timing()["sweep_ms"]
+            wall += time.perf_counter() - t0
+    plain_rate = integ / wall
+    log(f"batched: {len(probs)} problems, {iters} iterations in {t['run_ms']:.1f} ms "
+        f"({batched_rate:.3e} integrations/s); plain per-problem {wall * 1e3:.1f} ms wall "
+        f"({plain_rate:.3e}/s, device {dev_ms:.1f} ms): {batched_rate / plain_rate:.2f}x")
+    return {"workload": "64 fixed-source problems: groups 8/32/64/128, 2D segments per track "
+                        "U(5,500), 1-30 fine axial intervals, seeded; " + f"{iters} iterations",
+            "tracks": int(sum(p.inp.ntracks for p in probs)), "tally": tally,
+            "batched_ms": t["run_ms"], "batched_integrations_per_s": batched_rate,
+            "plain_wall_ms": wall * 1e3, "plain_device_ms": dev_ms,
+            "plain_integrations_per_s": plain_rate,
+            "batched_over_plain": batched_rate / plain_rate,
+            "batched_over_plain_device": batched_rate / (integ / (dev_ms * 1e-3)),
+            "tune_ms": tune["tune_ms"], "work_items": tune["n_items"],
+            "record_capacity": tune["record_capacity"],
+            "tuning": {str(k): v for k, v in sorted(groups.items())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +215,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched/plain leg")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only time a plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -283,6 +330,11 @@ def main():
             line["config"]["adjoint"] = adjoint_leg(sm)
         except Exception as e:  # reported, never fatal to the plain-sweep measurement
             log(f"adjoint leg failed: {e}")
+    if not args.no_batched:
+        try:
+            line["config"]["batched"] = batched_leg(sm, args.tally)
+        except Exception as e:  # reported, never fatal to the plain-sweep measurement
+            log(f"batched leg failed: {e}")
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:

@@ -156,6 +156,8 @@ typedef struct moc_options {
     int lanes_per_track;         /* 0 auto; 8, 16, 32 or 64 */
     int device;
     int pieces_per_batch;        /* attenuation: pieces whose loads issue together (0 auto; 1, 2, 4) */
+    long long max_records;       /* piece records held at once (0: as device memory allows);
+                                    a sweep needing more runs in chunks of whole z-stacks */
 } moc_options;
 
 void moc_default_options(moc_options *o);
@@ -204,6 +206,7 @@ typedef struct moc_timing {
     double adjoint_kernel_ms;    /* moc_adjoint: the reverse-sweep kernel */
     long segments;               /* 3D segment pieces of the last sweep */
     long sweeps;                 /* sweeps done */
+    long chunks;                 /* record chunks of the last sweep */
 } moc_timing;
 int moc_get_timing(moc_ctx *ctx, moc_timing *t);
 
@@ -220,8 +223,10 @@ int moc_get_track_records(moc_ctx *ctx, long track, float *ds, float *zin,
  * the scalar-flux tally.  moc_adjoint returns J and its gradient with
  * respect to sigma_t [nsr][G] and to the quadratic source coefficients
  * (c0, c1, c2) [nsr*fai][G][3] of that sweep, by a reverse sweep that
- * recomputes the attenuation per chunk of pieces (FP32 arithmetic, FP64
- * accumulation).  The solver state (flux, psi, z, sweep count) is not
+ * recomputes every piece's attenuation (FP32 arithmetic, FP64
+ * accumulation); only the angular flux entering each piece of the tracks
+ * in flight is kept, in a workspace of (resident tracks) x (longest track)
+ * x G floats.  The solver state (flux, psi, z, sweep count) is not
  * changed.  With the table exponential the derivative of F is the cell
  * slope.  Up to 64 groups. */
 int moc_adjoint(moc_ctx *ctx, const float *weights, double *J, double *grad_sigma,
@@ -232,6 +237,56 @@ int moc_adjoint(moc_ctx *ctx, const float *weights, double *J, double *grad_sigm
 int moc_set_coefficients(moc_ctx *ctx, const float *coef3, const float *sigma);
 /* The coefficients the next sweep uses. */
 int moc_get_coefficients(moc_ctx *ctx, float *coef3, float *sigma);
+
+/* Upper bound on the 3D pieces any sweep of this problem produces, for
+ * any z state (src/solver.c:409-525 advances one axial plane per piece
+ * that does not end its 2D segment, and stops at the domain boundary):
+ * per (2D track i, polar j) stack, every stacked track has at most
+ * per_stack[i*n_polar + j] pieces (per_stack may be NULL); *total sums the
+ * bound over all tracks. */
+int moc_record_bounds(const moc_problem *p, long long *per_stack, long long *total);
+
+/* ---- batched fixed-source sweeps (no counterpart in the reference) ----
+ * Many independent problems (any mix of shapes and group counts) swept
+ * together.  One iteration sweeps every problem once with the semantics
+ * of moc_transport_sweep (hash region draws of (seed, sweep), psi carried
+ * over, flux tallied onto the old flux, z heights evolving); the sources
+ * are fixed.  The attenuation is one persistent launch per iteration that
+ * takes work items (runs of one problem's tracks) from a queue ordered by
+ * piece count, longest first; lanes per track, groups per lane and tracks
+ * per item are chosen per group count by timing the candidates at
+ * creation (autotune).  moc_batch_run can replay its iterations as a
+ * captured hipGraph: nothing in an iteration needs the host. */
+typedef struct moc_batch moc_batch;
+typedef struct moc_batch_options {
+    int tally;                   /* MOC_TALLY_F64 (default) or MOC_TALLY_F32 */
+    int exp_mode;                /* MOC_EXP_TABLE (default) or MOC_EXP_INTRINSIC */
+    int autotune;                /* 1 (default): time the launch variants at creation */
+    int use_graph;               /* 1 (default): runs replay a captured hipGraph */
+    int device;
+} moc_batch_options;
+typedef struct moc_batch_timing {
+    double run_ms;               /* last moc_batch_run, device time */
+    long long iterations;        /* iterations of the last run */
+    long long integrations;      /* pieces x groups of the last run, all problems */
+    double tune_ms;              /* creation-time trace and autotuning */
+    long long record_capacity;   /* piece records allocated (bound for any sweep) */
+    int n_items;                 /* work items per attenuation launch */
+} moc_batch_timing;
+void moc_batch_default_options(moc_batch_options *o);
+/* seeds[p] is problem p's region-draw seed (NULL: p + 1) */
+int moc_batch_create(const moc_problem *const *problems, const uint64_t *seeds, int n,
+                     const moc_batch_options *o, moc_batch **out);
+int moc_batch_run(moc_batch *b, int iterations);
+/* problem p's flux, source, psi and z heights (MOC_DL_* bits) */
+int moc_batch_download(moc_batch *b, int problem, moc_problem *dst, unsigned what);
+/* pieces swept per problem since creation, [n] */
+int moc_batch_segments(moc_batch *b, long long *segments);
+/* launch variant chosen for problem p */
+int moc_batch_tuning(moc_batch *b, int problem, int *lanes_per_track, int *groups_per_lane,
+                     int *pieces_per_batch, int *tracks_per_item);
+int moc_batch_get_timing(moc_batch *b, moc_batch_timing *t);
+int moc_batch_destroy(moc_batch *b);
 
 /* Forward angular flux of one track (G floats), renormalisation applied. */
 int moc_get_track_psi(moc_ctx *ctx, long track, float *psi);

@@ -280,11 +280,11 @@ class Oracle:
         n2 = L.orc_trace_stack(C.byref(self.p), C.byref(self.opts), i, j,
                                _ptr(z, C.c_float), buf, n)
         assert n2 == n
-        rec = np.zeros(n, dtype=[("track", "i8"), ("n", "i4"), ("p", "i4"),
-                                 ("zh", "f4"), ("ds", "f4"), ("qsr", "i8")])
-        for q in range(n):
-            b = buf[q]
-            rec[q] = (b.track, b.n, b.p, b.zh, b.ds, b.qsr)
+        dt = np.dtype({"names": [f[0] for f in OrcPiece._fields_],
+                       "formats": ["i8", "i4", "i4", "f4", "f4", "i8"],
+                       "offsets": [getattr(OrcPiece, f[0]).offset for f in OrcPiece._fields_],
+                       "itemsize": C.sizeof(OrcPiece)})
+        rec = np.frombuffer(buf, dtype=dt, count=n).copy()
         return rec, z
 
 

@@ -88,7 +88,7 @@ class _Options(C.Structure):
     _fields_ = [
         ("qsr_mode", C.c_int), ("seed", C.c_uint64), ("tally", C.c_int), ("exp_mode", C.c_int),
         ("boundary", C.c_int), ("keep_b_psi", C.c_int), ("lanes_per_track", C.c_int),
-        ("device", C.c_int), ("pieces_per_batch", C.c_int),
+        ("device", C.c_int), ("pieces_per_batch", C.c_int), ("max_records", C.c_longlong),
     ]
 
 
@@ -98,8 +98,19 @@ class _Timing(C.Structure):
         ("attenuate_ms", C.c_double), ("finalize_ms", C.c_double), ("boundary_ms", C.c_double),
         ("renorm_ms", C.c_double), ("update_ms", C.c_double), ("keff_ms", C.c_double),
         ("adjoint_ms", C.c_double), ("adjoint_kernel_ms", C.c_double),
-        ("segments", C.c_long), ("sweeps", C.c_long),
+        ("segments", C.c_long), ("sweeps", C.c_long), ("chunks", C.c_long),
     ]
+
+
+class _BatchOptions(C.Structure):
+    _fields_ = [("tally", C.c_int), ("exp_mode", C.c_int), ("autotune", C.c_int),
+                ("use_graph", C.c_int), ("device", C.c_int)]
+
+
+class _BatchTiming(C.Structure):
+    _fields_ = [("run_ms", C.c_double), ("iterations", C.c_longlong),
+                ("integrations", C.c_longlong), ("tune_ms", C.c_double),
+                ("record_capacity", C.c_longlong), ("n_items", C.c_int)]
 
 
 _lib = None
@@ -146,6 +157,16 @@ def lib():
             "moc_last_error": ([], C.c_char_p),
             "moc_device_count": ([C.POINTER(C.c_int)], C.c_int),
             "moc_copy_probe": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+            "moc_record_bounds": ([P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
+            "moc_batch_default_options": ([C.POINTER(_BatchOptions)], None),
+            "moc_batch_create": ([C.POINTER(P), C.POINTER(C.c_uint64), C.c_int,
+                                  C.POINTER(_BatchOptions), C.POINTER(C.c_void_p)], C.c_int),
+            "moc_batch_run": ([C.c_void_p, C.c_int], C.c_int),
+            "moc_batch_download": ([C.c_void_p, C.c_int, P, C.c_uint], C.c_int),
+            "moc_batch_segments": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+            "moc_batch_tuning": ([C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 4, C.c_int),
+            "moc_batch_get_timing": ([C.c_void_p, C.POINTER(_BatchTiming)], C.c_int),
+            "moc_batch_destroy": ([C.c_void_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -322,14 +343,25 @@ class Problem:
     def leakage(self):
         return self._p.contents.leakage
 
+    def record_bounds(self):
+        """(per-stack bound on the pieces of each stacked track [nt2*n_polar],
+        bound on the pieces of a whole sweep) for any z state."""
+        I = self._p.contents.inp
+        per = np.zeros(I.ntracks_2D * I.n_polar_angles, np.int64)
+        tot = C.c_longlong()
+        _check(lib().moc_record_bounds(self._p, per.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                       C.byref(tot)))
+        return per, tot.value
+
 
 class Options:
     def __init__(self, qsr="hash", seed=1, tally="f64", exp="table", boundary="reflective",
-                 keep_b_psi=False, lanes_per_track=0, device=0, pieces_per_batch=0):
+                 keep_b_psi=False, lanes_per_track=0, device=0, pieces_per_batch=0, max_records=0):
         self.qsr, self.seed, self.tally, self.exp = qsr, seed, tally, exp
         self.boundary, self.keep_b_psi = boundary, keep_b_psi
         self.lanes_per_track, self.device = lanes_per_track, device
         self.pieces_per_batch = pieces_per_batch
+        self.max_records = max_records
 
     def _struct(self):
         o = _Options()
@@ -343,6 +375,7 @@ class Options:
         o.lanes_per_track = self.lanes_per_track
         o.device = self.device
         o.pieces_per_batch = self.pieces_per_batch
+        o.max_records = self.max_records
         return o
 
 
@@ -507,3 +540,83 @@ def device_count() -> int:
     n = C.c_int()
     lib().moc_device_count(C.byref(n))
     return n.value
+
+
+class Batch:
+    """Many fixed-source problems swept together (include/simplemoc.h
+    moc_batch_*): one iteration sweeps every problem once, like
+    Solver.transport_sweep on each, in one persistent launch per iteration,
+    replayed as a hipGraph."""
+
+    def __init__(self, problems, seeds=None, tally="f64", exp="table", autotune=True,
+                 use_graph=True, device=0):
+        self.problems = list(problems)
+        n = len(self.problems)
+        o = _BatchOptions()
+        lib().moc_batch_default_options(C.byref(o))
+        o.tally = {"f64": TALLY_F64, "f32": TALLY_F32}[tally]
+        o.exp_mode = {"table": EXP_TABLE, "intrinsic": EXP_INTRINSIC}[exp]
+        o.autotune, o.use_graph, o.device = int(autotune), int(use_graph), device
+        arr = (C.POINTER(_Problem) * n)(*[p._p for p in self.problems])
+        sd = None
+        if seeds is not None:
+            sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        h = C.c_void_p()
+        _check(lib().moc_batch_create(arr, sd, n, C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().moc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, iterations: int):
+        _check(lib().moc_batch_run(self._h, iterations))
+
+    def segments(self):
+        out = np.zeros(len(self.problems), np.int64)
+        _check(lib().moc_batch_segments(self._h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return out
+
+    def state(self, p: int, psi: bool = True):
+        """Problem p's flux, psi and z heights (numpy copies)."""
+        prob = self.problems[p]
+        I = prob.inp
+        T, G = I.ntracks, I.n_egroups
+        nval = I.n_source_regions_per_node * I.fai * G
+        flux = np.zeros(nval, np.float32)
+        z = np.zeros(T, np.float32)
+        ps = np.zeros(T * G if psi else 1, np.float32)
+        dst = _Problem()
+        dst.inp = I
+        dst.fine_flux = flux.ctypes.data_as(_fp)
+        dst.z_height = z.ctypes.data_as(_fp)
+        what = DL_FLUX | DL_Z
+        if psi:
+            dst.f_psi = ps.ctypes.data_as(_fp)
+            what |= DL_PSI
+        _check(lib().moc_batch_download(self._h, p, C.byref(dst), what))
+        out = {"fine_flux": flux, "z_height": z}
+        if psi:
+            out["f_psi"] = ps
+        return out
+
+    def tuning(self, p: int) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().moc_batch_tuning(self._h, p, *[C.byref(x) for x in v]))
+        return dict(zip(("lanes_per_track", "groups_per_lane", "pieces_per_batch",
+                         "tracks_per_item"), [x.value for x in v]))
+
+    def timing(self) -> dict:
+        t = _BatchTiming()
+        _check(lib().moc_batch_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}

@@ -14,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -39,7 +40,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", f"-I{IN
 LIB_NAME = "libsimplemoc_hip.so"
 DRIVER_NAME = "simplemoc"
 
-HIP_SOURCES = ["moc_device.hip"]
+HIP_SOURCES = ["moc_device.hip", "moc_batch.hip"]
 CXX_SOURCES = ["moc_host.cpp", "moc_dropin.cpp"]
 
 
@@ -56,19 +57,21 @@ def _run(cmd, verbose):
 def build(verbose: bool = True) -> str:
     """Compile every source (always) and return the library path."""
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HIP_SOURCES:
         o = os.path.join(OBJ, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
-        objs.append(o)
+        jobs.append(([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], o))
     for src in CXX_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         o = os.path.join(OBJ, src + ".o")
-        _run(["g++", *CXX_FLAGS, "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-c", path,
-              "-o", o], verbose)
-        objs.append(o)
+        jobs.append((["g++", *CXX_FLAGS, "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", "-c",
+                      path, "-o", o], o))
+    # the translation units compile independently
+    with ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
+        list(ex.map(lambda j: _run(j[0], verbose), jobs))
+    objs = [o for _, o in jobs]
     lib = os.path.join(OUT, LIB_NAME)
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,

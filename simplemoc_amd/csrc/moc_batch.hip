@@ -1,0 +1,1018 @@
+// moc_batch.hip -- many independent fixed-source problems per launch
+// (no reference counterpart; each problem's sweep is src/solver.c:283-552).
+//
+// One iteration sweeps every problem once, with the semantics of
+// moc_transport_sweep: the region draws of sweep s hash (seed, s), psi
+// carries over per track, the scalar flux is tallied onto the old one and
+// the z heights evolve.  The sources are fixed, so the fit coefficients
+// are computed once.  Per iteration, for all problems together:
+//
+//   k_bseed       acc = flux                      (FP64 tally only)
+//   k_btrace<0>   one workgroup per (problem, z-stack): piece counts
+//   k_scan_*      piece offsets over every track of every problem
+//   k_btrace<1>   piece records into one buffer, z heights committed
+//   k_batt        persistent attenuation: workgroups pull work items
+//                 (a run of one problem's tracks) from a queue ordered by
+//                 piece count, longest first; each item runs the launch
+//                 variant (lanes per track, groups per lane, pieces per
+//                 batch) chosen for its problem
+//   k_bfinal      flux = (float)acc               (FP64 tally only)
+//   k_bstep       sweep counters and piece totals
+//
+// Nothing in an iteration needs the host: the record buffer is sized once
+// by a bound valid for any z state, and the sweep counters live on the
+// device, so moc_batch_run can replay the iterations as a captured
+// hipGraph.  Kernels only, no memset nodes.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "moc_internal.h"
+#include "moc_kernels.h"
+#include "moc_sweep.h"
+
+using namespace moc;
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return moc_set_error(MOC_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                                      \
+    } while (0)
+#define TRY(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != MOC_OK)         \
+            return rc_;            \
+    } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------
+// launch variants of the attenuation: lanes per track, groups per lane,
+// pieces whose loads issue together
+struct Variant {
+    int lpt, gpl, pb;
+};
+constexpr Variant VARIANTS[] = {
+    {8, 1, 4},  {16, 1, 4}, {32, 1, 4}, {64, 1, 4},   // 0-3
+    {8, 2, 4},  {16, 2, 4}, {32, 2, 4}, {64, 2, 4},   // 4-7
+    {8, 4, 2},  {16, 4, 2}, {32, 4, 2}, {64, 4, 2},   // 8-11
+    {64, 8, 1},                                       // 12
+};
+constexpr int N_VARIANTS = sizeof(VARIANTS) / sizeof(VARIANTS[0]);
+
+// variants that cover G groups without leaving half the lanes idle
+std::vector<int> candidates(int G)
+{
+    std::vector<int> v;
+    for (int q = 0; q < N_VARIANTS; q++) {
+        const int cover = VARIANTS[q].lpt * VARIANTS[q].gpl;
+        if (cover >= G && cover < 2 * G)
+            v.push_back(q);
+    }
+    if (v.empty())
+        for (int q = 0; q < N_VARIANTS; q++)
+            if (VARIANTS[q].lpt * VARIANTS[q].gpl >= G) {
+                v.push_back(q);
+                break;
+            }
+    return v;
+}
+
+// the single-problem context's choice: fewest lanes that hold G, up to 64
+int default_variant(int G)
+{
+    int lpt = G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 64;
+    int gpl = (G + lpt - 1) / lpt;
+    int g2 = 1;
+    while (g2 < gpl)
+        g2 *= 2;
+    for (int q = 0; q < N_VARIANTS; q++)
+        if (VARIANTS[q].lpt == lpt && VARIANTS[q].gpl == g2)
+            return q;
+    return -1;
+}
+
+struct BatchItem {
+    int problem;
+    int variant;                // index into VARIANTS
+    int quad;                   // axial_exp == 2
+    int count;                  // tracks in the item
+    long long first;            // first track (problem-local index)
+};
+
+// ---------------------------------------------------------------------
+// kernels
+
+__global__ void k_bprep(const float *__restrict__ src, const float *__restrict__ sig,
+                        float4 *__restrict__ coef, long long nval, int G, int fai, int quad, float dz)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
+         e += (long long)gridDim.x * blockDim.x)
+        coef[e] = fit_coefficients(src, sig, e, G, fai, quad, dz);
+}
+
+__global__ void k_bseed(const float *__restrict__ flux, double *__restrict__ acc, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        acc[e] = (double)flux[e];
+}
+
+__global__ void k_bfinal(const double *__restrict__ acc, float *__restrict__ flux, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        flux[e] = (float)acc[e];
+}
+
+// global stack s -> (problem, local stack); commit < 0 keeps the problem's own flag
+template <bool WRITE, int KPT>
+__global__ __launch_bounds__(256) void k_btrace(const TraceArgs *__restrict__ probs,
+                                                 const int *__restrict__ stack_prob,
+                                                 const long long *__restrict__ stack_first,
+                                                 long long nst, int commit)
+{
+    __shared__ int s_w[16];
+    for (long long s = blockIdx.x; s < nst; s += gridDim.x) {
+        const int p = stack_prob[s];
+        TraceArgs a = probs[p];
+        if (commit >= 0)
+            a.commit = commit;
+        const uint64_t skey = sweep_key(a.seed, *a.sweep_ctr);
+        trace_stack<WRITE, KPT>(a, s - stack_first[p], skey, s_w);
+    }
+}
+
+// exclusive scan of n ints into int64, in tiles of SCAN_TILE
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ long long block_scan_ll(long long v, long long *s_w, long long *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    long long x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_w[w] = x;
+    __syncthreads();
+    long long pre = 0, tot = 0;
+    for (int q = 0; q < nw; q++) {
+        const long long t = s_w[q];
+        pre += (q < w) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const int *__restrict__ cnt,
+                                                             long long *__restrict__ off,
+                                                             long long n, long long *__restrict__ tile_sum)
+{
+    __shared__ long long s_w[SCAN_THREADS / 64];
+    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    long long v[SCAN_ITEMS];
+    long long run = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        v[q] = (base + q < n) ? (long long)cnt[base + q] : 0;
+        run += v[q];
+    }
+    long long tot;
+    long long pre = block_scan_ll(run, s_w, &tot);
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        if (base + q < n)
+            off[base + q] = pre;
+        pre += v[q];
+    }
+    if (threadIdx.x == 0)
+        tile_sum[blockIdx.x] = tot;
+}
+
+// one workgroup: exclusive scan of the tile sums in place
+__global__ __launch_bounds__(1024) void k_scan_sums(long long *__restrict__ tile_sum, long long nt)
+{
+    __shared__ long long s_w[16];
+    long long carry = 0;
+    for (long long b = 0; b < nt; b += blockDim.x) {
+        const long long e = b + threadIdx.x;
+        const long long v = e < nt ? tile_sum[e] : 0;
+        long long tot;
+        const long long pre = block_scan_ll(v, s_w, &tot);
+        if (e < nt)
+            tile_sum[e] = carry + pre;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_add(long long *__restrict__ off, long long n, const long long *__restrict__ tile_sum)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        off[e] += tile_sum[e / SCAN_TILE];
+}
+
+__global__ void k_reset_head(unsigned *head) { if (threadIdx.x == 0 && blockIdx.x == 0) head[0] = 0u; }
+
+// sweep counters +1; per-problem piece totals += this sweep's pieces
+__global__ void k_bstep(long long *__restrict__ sweep, long long *__restrict__ segs,
+                        const long long *__restrict__ off, const long long *__restrict__ track_first,
+                        int nprob)
+{
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+        sweep[p] += 1;
+        segs[p] += off[track_first[p + 1]] - off[track_first[p]];
+    }
+}
+
+template <int LPT, int GPL, int PB, bool T64, int EXPM>
+__device__ __forceinline__ void run_item(const AttArgs &a, const float2 *s_tab, const BatchItem &w)
+{
+    constexpr int TPB = 256 / LPT;
+    const int l = threadIdx.x % LPT;
+    const int lane_base = (threadIdx.x % 64) - l;
+    const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
+                                 : (int)(threadIdx.x / LPT);
+    for (int r = slot; r < w.count; r += TPB) {
+        const long long t = w.first + r;
+        if (w.quad)
+            attenuate_track<LPT, GPL, true, T64, EXPM, PB>(a, s_tab, t, l, lane_base);
+        else
+            attenuate_track<LPT, GPL, false, T64, EXPM, PB>(a, s_tab, t, l, lane_base);
+    }
+}
+
+// Persistent attenuation: every workgroup takes the next item from the
+// queue until the queue is empty (every wave reaches that exit).
+template <bool T64, int EXPM>
+__global__ __launch_bounds__(256) void k_batt(const AttArgs *__restrict__ probs,
+                                              const BatchItem *__restrict__ items, int n_items,
+                                              unsigned *__restrict__ head)
+{
+    extern __shared__ float2 s_tab[];
+    __shared__ int s_item;
+    int loaded = -1;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_item = (int)atomicAdd(head, 1u);
+        __syncthreads();
+        // uniform across the workgroup: the item and its problem's
+        // arguments are then read with scalar loads into SGPRs
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items)
+            break;
+        const BatchItem w = items[it];
+        const AttArgs a = probs[w.problem];
+        if (w.problem != loaded) {
+            for (int e = threadIdx.x; e < a.N; e += blockDim.x)
+                s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
+            __syncthreads();
+            loaded = w.problem;
+        }
+        switch (w.variant) {
+        case 0: run_item<8, 1, 4, T64, EXPM>(a, s_tab, w); break;
+        case 1: run_item<16, 1, 4, T64, EXPM>(a, s_tab, w); break;
+        case 2: run_item<32, 1, 4, T64, EXPM>(a, s_tab, w); break;
+        case 3: run_item<64, 1, 4, T64, EXPM>(a, s_tab, w); break;
+        case 4: run_item<8, 2, 4, T64, EXPM>(a, s_tab, w); break;
+        case 5: run_item<16, 2, 4, T64, EXPM>(a, s_tab, w); break;
+        case 6: run_item<32, 2, 4, T64, EXPM>(a, s_tab, w); break;
+        case 7: run_item<64, 2, 4, T64, EXPM>(a, s_tab, w); break;
+        case 8: run_item<8, 4, 2, T64, EXPM>(a, s_tab, w); break;
+        case 9: run_item<16, 4, 2, T64, EXPM>(a, s_tab, w); break;
+        case 10: run_item<32, 4, 2, T64, EXPM>(a, s_tab, w); break;
+        case 11: run_item<64, 4, 2, T64, EXPM>(a, s_tab, w); break;
+        default: run_item<64, 8, 1, T64, EXPM>(a, s_tab, w); break;
+        }
+    }
+}
+
+int launch_grid(long long work, int threads, int cap = 65536)
+{
+    long long b = (work + threads - 1) / threads;
+    return (int)std::max(1LL, std::min(b, (long long)cap));
+}
+
+template <class Tp>
+int dalloc(Tp **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(Tp));
+    if (e != hipSuccess)
+        return moc_set_error(MOC_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s",
+                             count * sizeof(Tp), hipGetErrorString(e));
+    return MOC_OK;
+}
+
+int trace_kpt(int zs) { return zs <= 256 ? 1 : zs <= 512 ? 2 : zs <= 1024 ? 4 : 8; }
+
+// per-problem host record
+struct BProb {
+    moc_input in;
+    long long nt2, T, nsr, nval, nst;
+    int G, np, zs, fai, quad;
+    uint64_t seed;
+    // offsets into the concatenated buffers
+    long long seg2d_first, segoff_first, nt2_first, np_first, track_first, psi_first, val_first,
+        sig_first, table_first, stack_first;
+    int table_N;
+    float table_dx, table_maxv;
+    int variant, tracks_per_item;
+};
+
+}  // namespace
+
+struct moc_batch {
+    moc_batch_options opt;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::vector<BProb> pb;
+    int nprob = 0;
+    long long T_all = 0, nst_all = 0, nval_all = 0, psi_all = 0, rec_cap = 0;
+    int kpt = 1, trace_threads = 64, max_N = 0, cus = 256;
+    // device buffers
+    long long *d_segoff = nullptr;
+    float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
+    double *d_cosd = nullptr, *d_sind = nullptr;
+    float *d_pw = nullptr, *d_z = nullptr, *d_psi = nullptr;
+    float *d_src = nullptr, *d_sig = nullptr, *d_flux = nullptr, *d_table = nullptr;
+    float4 *d_coef = nullptr;
+    double *d_acc = nullptr;
+    int *d_cnt = nullptr;
+    long long *d_off = nullptr, *d_tile = nullptr;
+    float4 *d_rec = nullptr;
+    long long *d_sweep = nullptr, *d_segs = nullptr, *d_track_first = nullptr, *d_stack_first = nullptr;
+    int *d_stack_prob = nullptr;
+    TraceArgs *d_tr = nullptr;
+    AttArgs *d_at = nullptr;
+    BatchItem *d_items = nullptr, *d_tune_items = nullptr;
+    int n_items = 0, tune_items_cap = 0;
+    unsigned *d_head = nullptr, *d_err = nullptr;
+    // graph of the last captured iteration count
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_iters = 0;
+    moc_batch_timing timing{};
+};
+
+namespace {
+
+void free_batch(moc_batch *b)
+{
+    if (b->graph_exec)
+        (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->d_segoff, b->d_seglen, b->d_az, b->d_mu, b->d_cosd, b->d_sind, b->d_pw,
+                    b->d_z, b->d_psi, b->d_src, b->d_sig, b->d_flux, b->d_table, b->d_coef,
+                    b->d_acc, b->d_cnt, b->d_off, b->d_tile, b->d_rec, b->d_sweep, b->d_segs,
+                    b->d_track_first, b->d_stack_first, b->d_stack_prob, b->d_tr, b->d_at,
+                    b->d_items, b->d_tune_items, b->d_head, b->d_err};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    for (auto &e : b->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+}
+
+int check_err(moc_batch *b)
+{
+    unsigned err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, b->d_err, 4, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (err)
+        return moc_set_error(MOC_ERR_STATE, "batched sweep: device check failed (flags 0x%x%s%s%s%s)",
+                             err, (err & ERR_QSR_RANGE) ? ", region draw out of range" : "",
+                             (err & ERR_FINE_RANGE) ? ", negative fine interval" : "",
+                             (err & ERR_RECORD_CAP) ? ", record capacity exceeded" : "",
+                             (err & ERR_TRACE_RUNAWAY) ? ", axial trace runaway" : "");
+    return MOC_OK;
+}
+
+template <bool WRITE>
+void launch_btrace(moc_batch *b, int commit)
+{
+    const int grid = (int)std::min(b->nst_all, (long long)(1 << 20));
+#define MOC_BT(K)                                                                              \
+    hipLaunchKernelGGL((k_btrace<WRITE, K>), dim3(grid), dim3(b->trace_threads), 0, b->stream, \
+                       b->d_tr, b->d_stack_prob, b->d_stack_first, b->nst_all, commit)
+    switch (b->kpt) {
+    case 1: MOC_BT(1); break;
+    case 2: MOC_BT(2); break;
+    case 4: MOC_BT(4); break;
+    default: MOC_BT(8); break;
+    }
+#undef MOC_BT
+}
+
+void launch_scan(moc_batch *b)
+{
+    const long long n = b->T_all + 1;
+    const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, b->stream, b->d_cnt,
+                       b->d_off, n, b->d_tile);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, b->stream, b->d_tile, tiles);
+    hipLaunchKernelGGL(k_scan_add, dim3(launch_grid(n, 256)), dim3(256), 0, b->stream, b->d_off, n,
+                       b->d_tile);
+}
+
+void launch_batt(moc_batch *b, const BatchItem *items, int n_items)
+{
+    hipLaunchKernelGGL(k_reset_head, dim3(1), dim3(64), 0, b->stream, b->d_head);
+    // resident workgroups: 8 per CU (256 threads, table in LDS)
+    const int grid = std::max(1, std::min(n_items, b->cus * 8));
+    const size_t lds = (size_t)b->max_N * sizeof(float2);
+    const bool t64 = b->opt.tally == MOC_TALLY_F64;
+    const bool tab = b->opt.exp_mode == MOC_EXP_TABLE;
+#define MOC_BATT(T, E)                                                                        \
+    hipLaunchKernelGGL((k_batt<T, E>), dim3(grid), dim3(256), lds, b->stream, b->d_at, items, \
+                       n_items, b->d_head)
+    if (t64) {
+        if (tab)
+            MOC_BATT(true, MOC_EXP_TABLE);
+        else
+            MOC_BATT(true, MOC_EXP_INTRINSIC);
+    } else {
+        if (tab)
+            MOC_BATT(false, MOC_EXP_TABLE);
+        else
+            MOC_BATT(false, MOC_EXP_INTRINSIC);
+    }
+#undef MOC_BATT
+}
+
+// one iteration: every problem swept once (no host interaction)
+void enqueue_iteration(moc_batch *b)
+{
+    const bool t64 = b->opt.tally == MOC_TALLY_F64;
+    if (t64)
+        hipLaunchKernelGGL(k_bseed, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
+                           b->d_flux, b->d_acc, b->nval_all);
+    launch_btrace<false>(b, -1);
+    launch_scan(b);
+    launch_btrace<true>(b, -1);
+    launch_batt(b, b->d_items, b->n_items);
+    if (t64)
+        hipLaunchKernelGGL(k_bfinal, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
+                           b->d_acc, b->d_flux, b->nval_all);
+    hipLaunchKernelGGL(k_bstep, dim3(launch_grid(b->nprob, 64)), dim3(64), 0, b->stream, b->d_sweep,
+                       b->d_segs, b->d_off, b->d_track_first, b->nprob);
+}
+
+// Work items of the given problems: runs of tracks_per_item tracks (whole
+// workgroup rounds of the variant), ordered by piece count, longest first.
+std::vector<BatchItem> build_items(const moc_batch *b, const std::vector<int> &probs,
+                                   const std::vector<int> &cnt)
+{
+    std::vector<std::pair<long long, BatchItem>> v;
+    for (int p : probs) {
+        const BProb &q = b->pb[p];
+        const Variant &var = VARIANTS[q.variant];
+        const int per = (256 / var.lpt) * q.tracks_per_item;
+        for (long long t0 = 0; t0 < q.T; t0 += per) {
+            BatchItem w;
+            w.problem = p;
+            w.variant = q.variant;
+            w.quad = q.quad;
+            w.first = t0;
+            w.count = (int)std::min((long long)per, q.T - t0);
+            // longest track of the item times the groups each lane carries
+            long long mx = 0;
+            for (long long t = t0; t < t0 + w.count; t++)
+                mx = std::max(mx, (long long)cnt[q.track_first + t]);
+            v.push_back({mx * var.gpl, w});
+        }
+    }
+    std::stable_sort(v.begin(), v.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::vector<BatchItem> out;
+    out.reserve(v.size());
+    for (auto &x : v)
+        out.push_back(x.second);
+    return out;
+}
+
+int upload_items(moc_batch *b, const std::vector<BatchItem> &items, bool tune)
+{
+    BatchItem **dst = tune ? &b->d_tune_items : &b->d_items;
+    int &cap = tune ? b->tune_items_cap : b->n_items;
+    if (tune) {
+        if ((int)items.size() > cap) {
+            if (*dst)
+                HIP_TRY(hipFree(*dst));
+            *dst = nullptr;
+            TRY(dalloc(dst, items.size()));
+            cap = (int)items.size();
+        }
+    } else {
+        if (*dst)
+            HIP_TRY(hipFree(*dst));
+        *dst = nullptr;
+        TRY(dalloc(dst, items.size()));
+        cap = (int)items.size();
+    }
+    HIP_TRY(hipMemcpy(*dst, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+    return MOC_OK;
+}
+
+// Per group count: time every candidate variant and item size on that
+// group count's problems (one traced sweep, psi and flux restored after)
+// and keep the fastest.
+int autotune(moc_batch *b, const std::vector<int> &cnt)
+{
+    std::map<int, std::vector<int>> by_g;
+    for (int p = 0; p < b->nprob; p++)
+        by_g[b->pb[p].G].push_back(p);
+    float *psi_save = nullptr, *flux_save = nullptr;
+    TRY(dalloc(&psi_save, (size_t)b->psi_all));
+    if (dalloc(&flux_save, (size_t)b->nval_all) != MOC_OK) {
+        (void)hipFree(psi_save);
+        return MOC_ERR_NOMEM;
+    }
+    int rc = MOC_OK;
+    auto body = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(psi_save, b->d_psi, b->psi_all * 4, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(flux_save, b->d_flux, b->nval_all * 4, hipMemcpyDeviceToDevice, b->stream));
+        const int sizes[] = {1, 4};
+        for (auto &kv : by_g) {
+            float best = 1e30f;
+            int best_v = b->pb[kv.second[0]].variant, best_r = 1;
+            for (int v : candidates(kv.first)) {
+                for (int r : sizes) {
+                    for (int p : kv.second) {
+                        b->pb[p].variant = v;
+                        b->pb[p].tracks_per_item = r;
+                    }
+                    std::vector<BatchItem> items = build_items(b, kv.second, cnt);
+                    TRY(upload_items(b, items, true));
+                    float ms = 1e30f;
+                    for (int rep = 0; rep < 2; rep++) {
+                        HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+                        launch_batt(b, b->d_tune_items, (int)items.size());
+                        HIP_TRY(hipGetLastError());
+                        HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+                        HIP_TRY(hipEventSynchronize(b->ev[1]));
+                        float t = 0;
+                        HIP_TRY(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+                        ms = std::min(ms, t);
+                    }
+                    if (ms < best) {
+                        best = ms;
+                        best_v = v;
+                        best_r = r;
+                    }
+                }
+            }
+            for (int p : kv.second) {
+                b->pb[p].variant = best_v;
+                b->pb[p].tracks_per_item = best_r;
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(b->d_psi, psi_save, b->psi_all * 4, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_flux, flux_save, b->nval_all * 4, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        return MOC_OK;
+    };
+    rc = body();
+    (void)hipFree(psi_save);
+    (void)hipFree(flux_save);
+    return rc;
+}
+
+int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
+                 const moc_batch_options *o, moc_batch *b)
+{
+    b->opt = *o;
+    b->device = o->device;
+    b->nprob = n;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return moc_set_error(MOC_ERR_DEVICE, "no HIP device available (this library needs a gfx950 GPU)");
+    if (o->device < 0 || o->device >= ndev)
+        return moc_set_error(MOC_ERR_DEVICE, "device %d out of range (%d devices)", o->device, ndev);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, o->device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return moc_set_error(MOC_ERR_DEVICE, "device %d is %s; this library is built for gfx950",
+                             o->device, prop.gcnArchName);
+    HIP_TRY(hipSetDevice(o->device));
+    b->cus = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto &e : b->ev)
+        HIP_TRY(hipEventCreate(&e));
+
+    // shapes and offsets
+    long long n_seg2d_all = 0, segoff_all = 0, nt2_all = 0, np_all = 0, sig_all = 0, tab_all = 0;
+    int max_zs = 1;
+    b->pb.resize(n);
+    for (int p = 0; p < n; p++) {
+        const moc_problem *P = probs[p];
+        if (!P || !P->in.derived)
+            return moc_set_error(MOC_ERR_INVALID, "moc_batch_create: problem %d missing or not derived", p);
+        const moc_input &I = P->in;
+        if (I.axial_exp != 0 && I.axial_exp != 2)
+            return moc_set_error(MOC_ERR_INVALID, "problem %d: invalid axial expansion order %d", p,
+                                 I.axial_exp);
+        if ((size_t)P->table_N * sizeof(float2) > 65536)
+            return moc_set_error(MOC_ERR_INVALID, "problem %d: exponential table exceeds 64 KiB", p);
+        BProb &q = b->pb[p];
+        q.in = I;
+        q.nt2 = I.ntracks_2D;
+        q.T = I.ntracks;
+        q.nsr = I.n_source_regions_per_node;
+        q.G = I.n_egroups;
+        q.np = I.n_polar_angles;
+        q.zs = I.z_stacked;
+        q.fai = I.fai;
+        q.quad = I.axial_exp == 2;
+        q.nval = q.nsr * q.fai * q.G;
+        q.nst = q.nt2 * q.np;
+        if ((unsigned long long)q.nsr * (unsigned long long)q.fai >= (1ull << 32))
+            return moc_set_error(MOC_ERR_INVALID, "problem %d: too many flat source regions", p);
+        q.seed = seeds ? seeds[p] : (uint64_t)(p + 1);
+        q.variant = default_variant(q.G);
+        if (q.variant < 0)
+            return moc_set_error(MOC_ERR_INVALID, "problem %d: %d groups (at most 512)", p, q.G);
+        q.tracks_per_item = 1;
+        q.table_N = P->table_N;
+        q.table_dx = P->table_dx;
+        q.table_maxv = P->table_maxVal;
+        q.seg2d_first = n_seg2d_all;
+        q.segoff_first = segoff_all;
+        q.nt2_first = nt2_all;
+        q.np_first = np_all;
+        q.track_first = b->T_all;
+        q.psi_first = b->psi_all;
+        q.val_first = b->nval_all;
+        q.sig_first = sig_all;
+        q.table_first = tab_all;
+        q.stack_first = b->nst_all;
+        n_seg2d_all += P->n_seg2d;
+        segoff_all += q.nt2 + 1;
+        nt2_all += q.nt2;
+        np_all += q.np;
+        b->T_all += q.T;
+        b->psi_all += q.T * q.G;
+        b->nval_all += q.nval;
+        sig_all += q.nsr * q.G;
+        tab_all += 2LL * q.table_N;
+        b->nst_all += q.nst;
+        max_zs = std::max(max_zs, q.zs);
+        b->max_N = std::max(b->max_N, q.table_N);
+        long long bound = 0;
+        TRY(moc_record_bounds(P, nullptr, &bound));
+        b->rec_cap += bound;
+    }
+    if (max_zs > 2048)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_create: z stacks of %d tracks (at most 2048)",
+                             max_zs);
+    b->kpt = trace_kpt(max_zs);
+    b->trace_threads = (int)(((max_zs + b->kpt - 1) / b->kpt + 63) / 64 * 64);
+
+    TRY(dalloc(&b->d_segoff, segoff_all));
+    TRY(dalloc(&b->d_seglen, n_seg2d_all));
+    TRY(dalloc(&b->d_az, nt2_all));
+    TRY(dalloc(&b->d_mu, np_all));
+    TRY(dalloc(&b->d_cosd, np_all));
+    TRY(dalloc(&b->d_sind, np_all));
+    TRY(dalloc(&b->d_pw, b->T_all));
+    TRY(dalloc(&b->d_z, b->T_all));
+    TRY(dalloc(&b->d_psi, b->psi_all));
+    TRY(dalloc(&b->d_src, b->nval_all));
+    TRY(dalloc(&b->d_sig, sig_all));
+    TRY(dalloc(&b->d_flux, b->nval_all));
+    TRY(dalloc(&b->d_table, tab_all));
+    TRY(dalloc(&b->d_coef, b->nval_all));
+    if (o->tally == MOC_TALLY_F64)
+        TRY(dalloc(&b->d_acc, b->nval_all));
+    TRY(dalloc(&b->d_cnt, b->T_all + 1));
+    TRY(dalloc(&b->d_off, b->T_all + 1));
+    TRY(dalloc(&b->d_tile, (b->T_all + 1 + SCAN_TILE - 1) / SCAN_TILE + 1));
+    TRY(dalloc(&b->d_rec, b->rec_cap));
+    TRY(dalloc(&b->d_sweep, n));
+    TRY(dalloc(&b->d_segs, n));
+    TRY(dalloc(&b->d_track_first, n + 1));
+    TRY(dalloc(&b->d_stack_first, n + 1));
+    TRY(dalloc(&b->d_stack_prob, b->nst_all));
+    TRY(dalloc(&b->d_tr, n));
+    TRY(dalloc(&b->d_at, n));
+    TRY(dalloc(&b->d_head, 1));
+    TRY(dalloc(&b->d_err, 1));
+
+    // uploads
+    std::vector<TraceArgs> tr(n);
+    std::vector<AttArgs> at(n);
+    std::vector<long long> track_first(n + 1), stack_first(n + 1);
+    std::vector<int> stack_prob(b->nst_all);
+    for (int p = 0; p < n; p++) {
+        const moc_problem *P = probs[p];
+        BProb &q = b->pb[p];
+        std::vector<long long> segoff(q.nt2 + 1);
+        for (long long i = 0; i <= q.nt2; i++)
+            segoff[i] = P->seg_offset[i];
+        std::vector<float> mu(q.np);
+        std::vector<double> cosd(q.np), sind(q.np);
+        for (int j = 0; j < q.np; j++) {
+            cosd[j] = cos((double)P->polar_angle[j]);
+            sind[j] = sin((double)P->polar_angle[j]);
+            mu[j] = (float)cosd[j];
+        }
+        HIP_TRY(hipMemcpy(b->d_segoff + q.segoff_first, segoff.data(), (q.nt2 + 1) * 8, hipMemcpyHostToDevice));
+        if (P->n_seg2d)
+            HIP_TRY(hipMemcpy(b->d_seglen + q.seg2d_first, P->seg_length, P->n_seg2d * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_az + q.nt2_first, P->az_weight, q.nt2 * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_mu + q.np_first, mu.data(), q.np * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_cosd + q.np_first, cosd.data(), q.np * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_sind + q.np_first, sind.data(), q.np * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_pw + q.track_first, P->p_weight, q.T * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_z + q.track_first, P->z_height, q.T * 4, hipMemcpyHostToDevice));
+        if (P->f_psi)
+            HIP_TRY(hipMemcpy(b->d_psi + q.psi_first, P->f_psi, q.T * q.G * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_src + q.val_first, P->fine_source, q.nval * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_flux + q.val_first, P->fine_flux, q.nval * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_sig + q.sig_first, P->sigT, q.nsr * q.G * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->d_table + q.table_first, P->table, 2 * (size_t)q.table_N * 4, hipMemcpyHostToDevice));
+        if (!P->f_psi)
+            HIP_TRY(hipMemset(b->d_psi + q.psi_first, 0, q.T * q.G * 4));
+
+        TraceArgs &a = tr[p];
+        memset(&a, 0, sizeof(a));
+        a.seg_off = b->d_segoff + q.segoff_first;
+        a.seg_len = b->d_seglen + q.seg2d_first;
+        a.cosd = b->d_cosd + q.np_first;
+        a.sind = b->d_sind + q.np_first;
+        a.z = b->d_z + q.track_first;
+        a.cnt = b->d_cnt + q.track_first;
+        a.off = b->d_off + q.track_first;
+        a.rec = b->d_rec;
+        a.rec_cap = b->rec_cap;
+        a.err = b->d_err;
+        a.nt2 = q.nt2;
+        a.np = q.np;
+        a.zs = q.zs;
+        a.fai = q.fai;
+        a.quad = q.quad;
+        a.nsr = q.nsr;
+        // src/solver.c:288-289 and :38
+        a.node_dz = (double)(q.in.height / (float)q.in.decomp_assemblies_ax);
+        a.fine_dz = a.node_dz / (double)(q.in.cai * q.fai);
+        a.dzf = (float)a.fine_dz;
+        a.dz_att = q.in.height / (float)(q.fai * q.in.decomp_assemblies_ax * q.in.cai);
+        a.sep = q.in.axial_z_sep;
+        a.seed = q.seed;
+        a.sweep_ctr = b->d_sweep + p;
+        a.commit = 1;
+
+        AttArgs &t = at[p];
+        memset(&t, 0, sizeof(t));
+        t.off = b->d_off + q.track_first;
+        t.rec = b->d_rec;
+        t.coef = b->d_coef + q.val_first;
+        t.table = b->d_table + q.table_first;
+        t.psi = b->d_psi + q.psi_first;
+        t.psi_scale = nullptr;
+        t.acc64 = b->d_acc ? b->d_acc + q.val_first : nullptr;
+        t.flux32 = b->d_acc ? nullptr : b->d_flux + q.val_first;
+        t.pw = b->d_pw + q.track_first;
+        t.az = b->d_az + q.nt2_first;
+        t.mu = b->d_mu + q.np_first;
+        t.T = q.T;
+        t.G = q.G;
+        t.np = q.np;
+        t.zs = q.zs;
+        t.N = q.table_N;
+        t.dx = q.table_dx;
+        t.hdx = 0.5f * q.table_dx;
+        t.maxv = q.table_maxv;
+
+        track_first[p] = q.track_first;
+        stack_first[p] = q.stack_first;
+        for (long long s = 0; s < q.nst; s++)
+            stack_prob[q.stack_first + s] = p;
+    }
+    track_first[n] = b->T_all;
+    stack_first[n] = b->nst_all;
+    HIP_TRY(hipMemcpy(b->d_tr, tr.data(), n * sizeof(TraceArgs), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_at, at.data(), n * sizeof(AttArgs), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_track_first, track_first.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_stack_first, stack_first.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_stack_prob, stack_prob.data(), b->nst_all * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(b->d_sweep, 0, n * 8));
+    HIP_TRY(hipMemset(b->d_segs, 0, n * 8));
+    HIP_TRY(hipMemset(b->d_cnt, 0, (b->T_all + 1) * 4));
+    HIP_TRY(hipMemset(b->d_err, 0, 4));
+    HIP_TRY(hipDeviceSynchronize());
+    // coefficients of the fixed sources
+    for (int p = 0; p < n; p++) {
+        const BProb &q = b->pb[p];
+        hipLaunchKernelGGL(k_bprep, dim3(launch_grid(q.nval, 256)), dim3(256), 0, b->stream,
+                           b->d_src + q.val_first, b->d_sig + q.sig_first, b->d_coef + q.val_first,
+                           q.nval, q.G, q.fai, q.quad,
+                           q.in.height / (float)(q.fai * q.in.decomp_assemblies_ax * q.in.cai));
+        HIP_TRY(hipGetLastError());
+    }
+
+    // piece counts of the first sweep order the work queue (and feed the tuner)
+    HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+    launch_btrace<false>(b, 0);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> cnt(b->T_all + 1);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), b->d_cnt, (b->T_all + 1) * 4, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (o->autotune) {
+        launch_scan(b);
+        launch_btrace<true>(b, 0);   // records of the first sweep, z untouched
+        HIP_TRY(hipGetLastError());
+        TRY(check_err(b));
+        TRY(autotune(b, cnt));
+    }
+    std::vector<int> all(n);
+    for (int p = 0; p < n; p++)
+        all[p] = p;
+    std::vector<BatchItem> items = build_items(b, all, cnt);
+    TRY(upload_items(b, items, false));
+    HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[3]));
+    float tune_ms = 0;
+    HIP_TRY(hipEventElapsedTime(&tune_ms, b->ev[2], b->ev[3]));
+    b->timing.tune_ms = tune_ms;
+    b->timing.record_capacity = b->rec_cap;
+    b->timing.n_items = b->n_items;
+    return check_err(b);
+}
+
+}  // namespace
+
+extern "C" void moc_batch_default_options(moc_batch_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->tally = MOC_TALLY_F64;
+    o->exp_mode = MOC_EXP_TABLE;
+    o->autotune = 1;
+    o->use_graph = 1;
+    o->device = 0;
+}
+
+extern "C" int moc_batch_create(const moc_problem *const *problems, const uint64_t *seeds, int n,
+                                const moc_batch_options *o_in, moc_batch **out)
+{
+    if (!problems || !out || n < 1)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_create: no problems");
+    moc_batch_options o;
+    if (o_in)
+        o = *o_in;
+    else
+        moc_batch_default_options(&o);
+    // as moc_create: the HIP runtime's initialisation must not move the
+    // caller's libc rand() sequence
+    static char private_state[256];
+    char *callers_state = initstate(1u, private_state, sizeof(private_state));
+    moc_batch *b = new moc_batch();
+    int rc = create_batch(problems, seeds, n, &o, b);
+    setstate(callers_state);
+    if (rc != MOC_OK) {
+        free_batch(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_run(moc_batch *b, int iterations)
+{
+    if (!b || iterations < 0)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_run: bad argument");
+    HIP_TRY(hipSetDevice(b->device));
+    if (iterations == 0)
+        return MOC_OK;
+    if (b->opt.use_graph && b->graph_iters != iterations) {
+        if (b->graph_exec)
+            HIP_TRY(hipGraphExecDestroy(b->graph_exec));
+        if (b->graph)
+            HIP_TRY(hipGraphDestroy(b->graph));
+        b->graph_exec = nullptr;
+        b->graph = nullptr;
+        b->graph_iters = 0;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < iterations; it++)
+            enqueue_iteration(b);
+        HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+        HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        b->graph_iters = iterations;
+    }
+    std::vector<long long> before(b->nprob), after(b->nprob);
+    HIP_TRY(hipMemcpyAsync(before.data(), b->d_segs, b->nprob * 8, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    if (b->opt.use_graph) {
+        HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+    } else {
+        for (int it = 0; it < iterations; it++)
+            enqueue_iteration(b);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    TRY(check_err(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    HIP_TRY(hipMemcpy(after.data(), b->d_segs, b->nprob * 8, hipMemcpyDeviceToHost));
+    long long integ = 0;
+    for (int p = 0; p < b->nprob; p++)
+        integ += (after[p] - before[p]) * b->pb[p].G;
+    b->timing.run_ms = ms;
+    b->timing.iterations = iterations;
+    b->timing.integrations = integ;
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_download(moc_batch *b, int problem, moc_problem *dst, unsigned what)
+{
+    if (!b || !dst || problem < 0 || problem >= b->nprob)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_download: bad argument");
+    const BProb &q = b->pb[problem];
+    if (dst->in.ntracks != q.T || dst->in.n_egroups != q.G ||
+        dst->in.n_source_regions_per_node != q.nsr || dst->in.fai != q.fai)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_download: problem shape differs");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((what & MOC_DL_FLUX) && dst->fine_flux)
+        HIP_TRY(hipMemcpy(dst->fine_flux, b->d_flux + q.val_first, q.nval * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_SOURCE) && dst->fine_source)
+        HIP_TRY(hipMemcpy(dst->fine_source, b->d_src + q.val_first, q.nval * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_PSI) && dst->f_psi)
+        HIP_TRY(hipMemcpy(dst->f_psi, b->d_psi + q.psi_first, q.T * q.G * 4, hipMemcpyDeviceToHost));
+    if ((what & MOC_DL_Z) && dst->z_height)
+        HIP_TRY(hipMemcpy(dst->z_height, b->d_z + q.track_first, q.T * 4, hipMemcpyDeviceToHost));
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_segments(moc_batch *b, long long *segments)
+{
+    if (!b || !segments)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_segments: null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(segments, b->d_segs, b->nprob * 8, hipMemcpyDeviceToHost));
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_tuning(moc_batch *b, int problem, int *lanes_per_track, int *groups_per_lane,
+                                int *pieces_per_batch, int *tracks_per_item)
+{
+    if (!b || problem < 0 || problem >= b->nprob)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_tuning: bad argument");
+    const BProb &q = b->pb[problem];
+    const Variant &v = VARIANTS[q.variant];
+    if (lanes_per_track)
+        *lanes_per_track = v.lpt;
+    if (groups_per_lane)
+        *groups_per_lane = v.gpl;
+    if (pieces_per_batch)
+        *pieces_per_batch = v.pb;
+    if (tracks_per_item)
+        *tracks_per_item = (256 / v.lpt) * q.tracks_per_item;
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_get_timing(moc_batch *b, moc_batch_timing *t)
+{
+    if (!b || !t)
+        return moc_set_error(MOC_ERR_INVALID, "moc_batch_get_timing: null argument");
+    *t = b->timing;
+    return MOC_OK;
+}
+
+extern "C" int moc_batch_destroy(moc_batch *b)
+{
+    if (!b)
+        return MOC_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    delete b;
+    return MOC_OK;
+}

@@ -29,6 +29,7 @@
 
 #include "moc_internal.h"
 #include "moc_kernels.h"
+#include "moc_sweep.h"
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -43,31 +44,6 @@ using namespace moc;
             return moc_set_error(MOC_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                                  __FILE__, __LINE__);                                      \
     } while (0)
-
-// ---------------------------------------------------------------------
-// hashing of region draws (same definition as the oracle's qsr hash)
-__host__ __device__ static inline uint64_t mix64(uint64_t z)
-{
-    z ^= z >> 30;
-    z *= 0xBF58476D1CE4E5B9ULL;
-    z ^= z >> 27;
-    z *= 0x94D049BB133111EBULL;
-    z ^= z >> 31;
-    return z;
-}
-static inline uint64_t sweep_key(uint64_t seed, long sweep)
-{
-    return mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sweep + 1));
-}
-__device__ static inline uint64_t track_key(uint64_t skey, long long t)
-{
-    return mix64(skey ^ ((uint64_t)t * 0xD1B54A32D192ED03ULL));
-}
-__device__ static inline uint32_t piece_region(uint64_t tkey, long long n, int p, long long nsr)
-{
-    uint64_t h = mix64(tkey + (((uint64_t)n << 32) | (uint64_t)(uint32_t)p));
-    return (uint32_t)(((h >> 32) * (uint64_t)nsr) >> 32);
-}
 
 // ---------------------------------------------------------------------
 // exact reference pairwise sum (src/utils.c:29-45) over an element functor:
@@ -95,387 +71,32 @@ __device__ inline float pairwise(const F &f, long long base, long long n)
 }
 
 // ---------------------------------------------------------------------
-// (1) source-fit coefficients per (FSR, group) + FP64 accumulator seed.
-// Edge rule of src/solver.c:55-138 with row indices clamped into
-// [0, fai-1] (a no-op for fai >= 3; defined fallback for fai < 3).
+// (1)-(3) the sweep kernels of one problem (device code in moc_sweep.h)
+
+// coefficients, and the FP64 tally accumulator seeded with the old flux
 __global__ void k_prep(const float *__restrict__ src, const float *__restrict__ sig,
                        const float *__restrict__ flux, float4 *__restrict__ coef,
                        double *__restrict__ acc, long long nval, int G, int fai, int quad,
                        float dz)
 {
-    const float two_dz = 2.f * dz;
-    const float two_dz2 = 2.f * dz * dz;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nval;
          e += (long long)gridDim.x * blockDim.x) {
-        long long fsr = e / G;
-        int g = (int)(e - fsr * G);
-        long long r = fsr / fai;
-        int fine = (int)(fsr - r * fai);
-        float s = sig[r * G + g];
-        float4 c;
-        if (quad) {
-            int r1, r2, r3;
-            if (fine == 0) {
-                r1 = 0; r2 = 1; r3 = 2;
-            } else if (fine == fai - 1) {
-                r1 = fine - 2; r2 = fine - 1; r3 = fine;
-            } else {
-                r1 = fine - 1; r2 = fine; r3 = fine + 1;
-            }
-            r1 = r1 < 0 ? 0 : (r1 > fai - 1 ? fai - 1 : r1);
-            r2 = r2 < 0 ? 0 : (r2 > fai - 1 ? fai - 1 : r2);
-            r3 = r3 > fai - 1 ? fai - 1 : r3;
-            const float *base = src + r * fai * G + g;
-            float y1 = base[(long long)r1 * G];
-            float y2 = base[(long long)r2 * G];
-            float y3 = base[(long long)r3 * G];
-            c.x = y2;
-            c.y = (y1 - y3) / two_dz;
-            c.z = (y1 - 2.f * y2 + y3) / two_dz2;
-        } else {
-            c.x = src[e];
-            c.y = 0.f;
-            c.z = 0.f;
-        }
-        c.w = s;
-        coef[e] = c;
+        coef[e] = fit_coefficients(src, sig, e, G, fai, quad, dz);
         if (acc)
             acc[e] = (double)flux[e];
     }
 }
 
-// ---------------------------------------------------------------------
-// (2) axial ray trace (src/solver.c:366-527).  One workgroup per
-// (2D track, polar angle) z-stack; thread `tid` owns the stacked tracks
-// k = tid*KPT .. tid*KPT+KPT-1 with their z heights in registers.
-//
-// The reference walks k serially inside each 2D segment n and changes the
-// loop bounds while it runs: going up, every exit does end-- before the
-// next k is tested; going down, begin++ only affects later segments.  Here
-// every active track first traces segment n tentatively (pieces, exit,
-// final z).  Going up, track k is processed iff k < end - E(k), with E(k)
-// the exits of the lower tracks; that test fails monotonically, so the
-// processed tracks are a prefix [begin, kstop) found with one block scan
-// and one block minimum.  Processed tracks then commit (and, in the write
-// pass, re-trace to emit records); the others keep their z.  The result
-// equals the serial loop, including the quirk of retiring the wrong track
-// when several exit in one segment.
-struct SegResult {
-    int pieces;
-    int exited;
-    float zfin;
-};
-
-struct EmitCtx {
-    float4 *rec;
-    long long rec_cap;
-    long long base;             // record index of the track's next piece
-    const int32_t *stream;      // explicit region draws (or null)
-    long long serial;           // serial index of the first piece (stream mode)
-    uint64_t tkey;
-    long long nsr;
-    int fai;
-    int quad;
-    float dz_att;
-    unsigned *err;
-};
-
-// One (track, 2D segment): the while loop of src/solver.c:409-525.
-template <bool EMIT>
-__device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k, bool pos,
-                                                   bool last_seg, long long n, const TraceArgs &a,
-                                                   double c, EmitCtx *ec)
-{
-    float sr = s_full;
-    int curr = pos ? (int)(zk / a.dzf) : (int)ceil((double)(zk / a.dzf));
-    bool complete = false;
-    int p = 0;
-    int exited = 0;
-    while (!complete) {
-        bool reset = false;
-        float ds;
-        float zn = (float)((double)zk + (double)sr * c);
-        int ni = pos ? (int)(zn / a.dzf) : (int)ceil((double)(zn / a.dzf));
-        if (ni == curr) {
-            complete = true;
-            ds = sr;
-        } else {
-            curr += pos ? 1 : -1;
-            zn = (float)(a.fine_dz * (double)(float)curr);
-            ds = (float)((double)(zn - zk) / c);
-            sr -= ds;
-            if (sr <= 0.f)
-                complete = true;
-            if (zn <= 0.f || (double)zn >= a.node_dz) {
-                complete = true;
-                exited = 1;
-                reset = true;
-            }
-        }
-        if (EMIT) {
-            uint32_t qsr;
-            if (ec->stream) {
-                int32_t v = ec->stream[ec->serial + p];
-                if (v < 0 || (long long)v >= ec->nsr) {
-                    atomicOr(ec->err, ERR_QSR_RANGE);
-                    v = 0;
-                }
-                qsr = (uint32_t)v;
-            } else {
-                qsr = piece_region(ec->tkey, n, p, ec->nsr);
-            }
-            // cell of the piece start, as attenuate_fluxes computes it (:38-45)
-            const int cell = (int)(zk / ec->dz_att);
-            float zin = zk - ec->dz_att * ((float)cell + 0.5f);
-            int fine = cell % ec->fai;
-            if (ec->quad) {
-                if (fine == 0)
-                    zin -= ec->dz_att;
-                else if (fine == ec->fai - 1)
-                    zin += ec->dz_att;
-            }
-            if (fine < 0) {
-                atomicOr(ec->err, ERR_FINE_RANGE);
-                fine = 0;
-            }
-            const long long q = ec->base + p;
-            if (q < ec->rec_cap)
-                ec->rec[q] = make_float4(ds, zin, __uint_as_float(qsr * (uint32_t)ec->fai + (uint32_t)fine), 0.f);
-            else
-                atomicOr(ec->err, ERR_RECORD_CAP);
-        }
-        p++;
-        if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
-            atomicOr(a.err, ERR_TRACE_RUNAWAY);
-            complete = true;
-        }
-        if (last_seg || reset)
-            zk = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
-        else
-            zk = zn;
-    }
-    SegResult r;
-    r.pieces = p;
-    r.exited = exited;
-    r.zfin = zk;
-    return r;
-}
-
-// block-wide helpers (blockDim.x a multiple of 64, at most 1024)
-__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o)
-            x += y;
-    }
-    if (lane == 63)
-        s_w[w] = x;
-    __syncthreads();
-    int pre = 0, tot = 0;
-    for (int q = 0; q < nw; q++) {
-        int t = s_w[q];
-        pre += (q < w) ? t : 0;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + x - v;
-}
-
-__device__ __forceinline__ int block_min(int v, int *s_w)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = min(v, __shfl_xor(v, o, 64));
-    if (lane == 0)
-        s_w[w] = v;
-    __syncthreads();
-    int m = s_w[0];
-    for (int q = 1; q < nw; q++)
-        m = min(m, s_w[q]);
-    __syncthreads();
-    return m;
-}
-
+// one workgroup per z-stack
 template <bool WRITE, int KPT>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a)
 {
     __shared__ int s_w[16];
-    const long long nst = a.nt2 * a.np;
-    const int zs = a.zs;
-    const int k0 = threadIdx.x * KPT;
-    for (long long s = blockIdx.x; s < nst; s += gridDim.x) {
-        const long long i = s / a.np;
-        const int j = (int)(s - i * a.np);
-        const long long t0 = s * zs;
-        const bool pos = j < a.np / 2;
-        const double c = a.cosd[j];
-        const double sn = a.sind[j];
-        float z[KPT];
-        int cnt[KPT];
-        long long base[KPT];
-        uint64_t tkey[KPT];
-#pragma unroll
-        for (int r = 0; r < KPT; r++) {
-            const int k = k0 + r;
-            z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
-            cnt[r] = 0;
-            base[r] = (WRITE && k < zs) ? a.off[t0 + k] : 0;
-            tkey[r] = (WRITE && !a.qsr_stream) ? track_key(a.sweep_key, t0 + k) : 0;
-        }
-        int begin = 0, end = zs;
-        long long serial = (WRITE && a.qsr_stream) ? a.stack_off[s] : 0;
-        const long long off = a.seg_off[i];
-        const long long nseg = a.seg_off[i + 1] - off;
-        for (long long n = 0; n < nseg; n++) {
-            const float s_full = (float)((double)a.seg_len[off + n] / sn);
-            const bool last = (n == nseg - 1);
-            SegResult res[KPT];
-            bool act[KPT];
-            int nx = 0;
-#pragma unroll
-            for (int r = 0; r < KPT; r++) {
-                const int k = k0 + r;
-                act[r] = (k >= begin && k < end);
-                if (act[r]) {
-                    res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
-                } else {
-                    res[r].pieces = 0;
-                    res[r].exited = 0;
-                    res[r].zfin = z[r];
-                }
-                nx += res[r].exited;
-            }
-            // which active tracks the serial loop reaches in this segment
-            bool proc[KPT];
-            int ex_total;
-            int ex_pre = block_excl_scan(nx, s_w, &ex_total);
-            if (pos) {
-                int fail = INT_MAX;
-                int e = ex_pre;
-#pragma unroll
-                for (int r = 0; r < KPT; r++) {
-                    const int k = k0 + r;
-                    if (act[r] && k >= end - e && fail == INT_MAX)
-                        fail = k;
-                    e += res[r].exited;
-                }
-                const int kstop = (ex_total == 0) ? INT_MAX : block_min(fail, s_w);
-                int nproc_exits = 0;
-#pragma unroll
-                for (int r = 0; r < KPT; r++) {
-                    proc[r] = act[r] && (k0 + r) < kstop;
-                    nproc_exits += proc[r] ? res[r].exited : 0;
-                }
-                if (kstop != INT_MAX) {
-                    int tot;
-                    (void)block_excl_scan(nproc_exits, s_w, &tot);
-                    end -= tot;
-                } else {
-                    end -= ex_total;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < KPT; r++)
-                    proc[r] = act[r];
-                begin += ex_total;
-            }
-            // serial index of each track's first piece (region stream)
-            long long ser = 0;
-            if (WRITE && a.qsr_stream) {
-                int np_loc = 0;
-#pragma unroll
-                for (int r = 0; r < KPT; r++)
-                    np_loc += proc[r] ? res[r].pieces : 0;
-                int ptot;
-                int ppre = block_excl_scan(np_loc, s_w, &ptot);
-                ser = serial + ppre;
-                serial += ptot;
-            }
-#pragma unroll
-            for (int r = 0; r < KPT; r++) {
-                if (!proc[r])
-                    continue;
-                if (WRITE) {
-                    EmitCtx ec;
-                    ec.rec = a.rec;
-                    ec.rec_cap = a.rec_cap;
-                    ec.base = base[r] + cnt[r];
-                    ec.stream = a.qsr_stream;
-                    ec.serial = ser;
-                    ec.tkey = tkey[r];
-                    ec.nsr = a.nsr;
-                    ec.fai = a.fai;
-                    ec.quad = a.quad;
-                    ec.dz_att = a.dz_att;
-                    ec.err = a.err;
-                    (void)trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
-                    ser += res[r].pieces;
-                }
-                cnt[r] += res[r].pieces;
-                z[r] = res[r].zfin;
-            }
-        }
-        long long stack_total = 0;
-#pragma unroll
-        for (int r = 0; r < KPT; r++) {
-            const int k = k0 + r;
-            if (k < zs) {
-                if (WRITE && a.commit)
-                    a.z[t0 + k] = z[r];
-                else
-                    a.cnt[t0 + k] = cnt[r];
-            }
-            stack_total += cnt[r];
-        }
-        if (!WRITE && a.stack_cnt) {
-            int tot;
-            (void)block_excl_scan((int)stack_total, s_w, &tot);
-            if (threadIdx.x == 0)
-                a.stack_cnt[s] = tot;
-        }
-        __syncthreads();
-    }
+    for (long long s = a.s_begin + blockIdx.x; s < a.s_end; s += gridDim.x)
+        trace_stack<WRITE, KPT>(a, s, a.sweep_key, s_w);
 }
 
-// ---------------------------------------------------------------------
-// (3) attenuation (src/solver.c:14-280 quadratic, :1040-1138 flat)
-// 1 - exp(-tau): the reference table (src/solver.c:1441-1464) or -expm1f.
-// The index clamp is branch-free: v <= 0 (and NaN) -> cell 0, which is the
-// reference's truncation for -1 < v < 0; v >= N-1 -> cell N-1 (never
-// reached below maxVal).
-template <int EXPM>
-__device__ __forceinline__ float exp_term(const float2 *tab, float tau, float dx, float hdx,
-                                          float maxv, int N)
-{
-    if (EXPM != MOC_EXP_TABLE)
-        return -expm1f(-tau);
-    const float v = tau / dx + hdx;
-    const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
-    const float2 sl = tab[iv];
-    const float F = sl.x * tau + sl.y;
-    return tau > maxv ? 1.0f : F;
-}
-
-// broadcast lane `src` of x to the lanes of its track (LPT lanes per track)
-template <int LPT>
-__device__ __forceinline__ float bcast(float x, int src_in_track, int lane_base)
-{
-    if (LPT == 64)
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src_in_track));
-    return __shfl(x, lane_base + src_in_track, 64);
-}
-
-// One track per LPT lanes; lane l owns groups g = l + r*LPT (r < GPL).
-// Records are read LPT at a time (one coalesced load) and broadcast; the
-// coefficients of PB pieces are requested together so their latency, and
-// the drain of earlier atomics the compiler waits for, is paid once per PB
-// pieces.
+// 256/LPT tracks per block
 template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
 __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
 {
@@ -483,106 +104,16 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
     for (int e = threadIdx.x; e < a.N; e += blockDim.x)
         s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
     __syncthreads();
-
-    constexpr int TPB = 256 / LPT;   // tracks per block
+    constexpr int TPB = 256 / LPT;
     const int l = threadIdx.x % LPT;
     const int lane_base = (threadIdx.x % 64) - l;
     const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
                                  : (int)(threadIdx.x / LPT);
-    const int G = a.G;
-    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
-    const int N = a.N;
-    const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
-    int gcl[GPL];
-#pragma unroll
-    for (int r = 0; r < GPL; r++)
-        gcl[r] = min(l + r * LPT, G - 1);   // clamped group for unconditional loads
-    const long long nblk = (a.T + TPB - 1) / TPB;
+    const long long nblk = (a.t_end - a.t_begin + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const long long t = b * TPB + slot;
-        if (t >= a.T)
-            continue;
-        const int j = (int)((t / a.zs) % a.np);
-        const long long i = t / ((long long)a.zs * a.np);
-        const float mu = a.mu[j];
-        const float mu2 = mu * mu;
-        const float azw = a.az[i];
-        const float weight = QUAD ? a.pw[t] * azw : a.pw[t] * azw * mu;
-        float psi[GPL];
-#pragma unroll
-        for (int r = 0; r < GPL; r++)
-            // a pending renormalisation (src/solver.c:1224) is applied here
-            psi[r] = a.psi[t * G + gcl[r]] * pscale;
-        const long long q0 = a.off[t], q1 = a.off[t + 1];
-        for (long long qc = q0; qc < q1; qc += LPT) {
-            const int nq = (int)min((long long)LPT, q1 - qc);
-            const float4 mine = a.rec[qc + min(l, nq - 1)];
-            for (int p = 0; p < nq; p += PB) {
-                float ds[PB], zin[PB];
-                uint32_t fsr[PB];
-                float4 cf[PB][GPL];
-#pragma unroll
-                for (int u = 0; u < PB; u++) {
-                    const int pu = min(p + u, nq - 1);
-                    ds[u] = bcast<LPT>(mine.x, pu, lane_base);
-                    zin[u] = bcast<LPT>(mine.y, pu, lane_base);
-                    fsr[u] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
-#pragma unroll
-                    for (int r = 0; r < GPL; r++)
-                        cf[u][r] = a.coef[(size_t)fsr[u] * G + gcl[r]];
-                }
-#pragma unroll
-                for (int u = 0; u < PB; u++) {
-                    if (p + u >= nq)
-                        break;
-#pragma unroll
-                    for (int r = 0; r < GPL; r++) {
-                        const int g = l + r * LPT;
-                        if (g >= G)
-                            continue;
-                        const float4 c = cf[u][r];
-                        const float sig = c.w;
-                        const float tau = sig * ds[u];
-                        const float F = exp_term<EXPM>(s_tab, tau, dx, hdx, maxv, N);
-                        const float ps = psi[r];
-                        float tally;
-                        if (QUAD) {
-                            const float zi = zin[u];
-                            const float q0v = c.x + c.y * zi + c.z * zi * zi;
-                            const float q1v = c.y + 2.f * c.z * zi;
-                            const float q2v = c.z;
-                            const float sig2 = sig * sig;
-                            const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * sig2);
-                            const float fi = (q0v * tau + (sig * ps - q0v) * F) / sig2 +
-                                             q1v * mu * reuse +
-                                             q2v * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) /
-                                                 (3.f * sig2 * sig2);
-                            tally = weight * fi;
-                            const float t1 = q0v * F / sig;
-                            const float t2 = q1v * mu * (tau - F) / sig2;
-                            const float t3 = q2v * mu2 * reuse;
-                            const float t4 = ps * (1.f - F);
-                            psi[r] = t1 + t2 + t3 + t4;
-                        } else {
-                            const float qv = c.x / sig;
-                            const float dpsi = (ps - qv) * F;
-                            tally = weight * dpsi;
-                            psi[r] = ps - dpsi;
-                        }
-                        if (T64)
-                            atomicAdd(a.acc64 + (size_t)fsr[u] * G + g, (double)tally);
-                        else
-                            atomicAdd(a.flux32 + (size_t)fsr[u] * G + g, tally);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < GPL; r++) {
-            const int g = l + r * LPT;
-            if (g < G)
-                a.psi[t * G + g] = psi[r];
-        }
+        const long long t = a.t_begin + b * TPB + slot;
+        if (t < a.t_end)
+            attenuate_track<LPT, GPL, QUAD, T64, EXPM, PB>(a, s_tab, t, l, lane_base);
     }
 }
 
@@ -628,6 +159,8 @@ struct AdjArgs {
     float dx;
     float hdx;
     float maxv;
+    long long rec_base;         // record index of rec[0]
+    long long t_begin, t_end;   // tracks of this launch
 };
 
 template <int EXPM>
@@ -753,18 +286,18 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
     const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
     float *tape = a.tape + ((long long)blockIdx.x * TPB + slot) * a.tape_stride + g;
     double Jacc = 0.0;
-    const long long nblk = (a.T + TPB - 1) / TPB;
+    const long long nblk = (a.t_end - a.t_begin + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const long long t = b * TPB + slot;
-        const bool has = t < a.T;
-        const long long tt = has ? t : 0;
+        const long long t = a.t_begin + b * TPB + slot;
+        const bool has = t < a.t_end;
+        const long long tt = has ? t : a.t_begin;
         const int j = (int)((tt / a.zs) % a.np);
         const long long i = tt / ((long long)a.zs * a.np);
         const float mu = a.mu[j];
         const float mu2 = mu * mu;
         const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
-        const long long q0 = has ? a.off[tt] : 0;
-        const int P = has ? (int)(a.off[tt + 1] - q0) : 0;
+        const long long q0 = has ? a.off[tt] - a.rec_base : 0;
+        const int P = has ? (int)(a.off[tt + 1] - a.off[tt]) : 0;
         float ps = a.psi[tt * G + g] * pscale;
         // forward: psi entering each piece onto the tape
         for (int p0 = 0; p0 < P; p0 += ADJ_PB) {
@@ -1222,6 +755,8 @@ struct moc_ctx {
     // bookkeeping
     long sweeps = 0;
     long last_segments = 0;
+    long chunks = 0;            // record chunks of the last sweep
+    long long rec_base = 0, rec_t_begin = 0, rec_t_end = 0;   // what d_rec holds
     moc_timing timing{};
     hipEvent_t ev[8] = {};
 };
@@ -1288,6 +823,7 @@ extern "C" void moc_default_options(moc_options *o)
     o->lanes_per_track = 0;
     o->device = 0;
     o->pieces_per_batch = 0;
+    o->max_records = 0;
 }
 
 extern "C" int moc_device_count(int *n)
@@ -1577,6 +1113,9 @@ static TraceArgs trace_args(moc_ctx *c)
     a.sep = c->in.axial_z_sep;
     a.sweep_key = sweep_key(c->opt.seed, c->sweeps);
     a.commit = 1;
+    a.rec_base = 0;
+    a.s_begin = 0;
+    a.s_end = c->nt2 * c->np;
     return a;
 }
 
@@ -1619,8 +1158,8 @@ static void launch_att_lpt(moc_ctx *c, const AttArgs &a, int grid, size_t lds)
 static void launch_attenuate(moc_ctx *c, const AttArgs &a)
 {
     const int tpb = 256 / c->lpt;
-    const long long nblk = (c->T + tpb - 1) / tpb;
-    const int grid = (int)(nblk < 1048576 ? nblk : 1048576);
+    const long long nblk = (a.t_end - a.t_begin + tpb - 1) / tpb;
+    const int grid = (int)(nblk < 1048576 ? (nblk < 1 ? 1 : nblk) : 1048576);
     const size_t lds = (size_t)c->table_N * sizeof(float2);
     switch (c->lpt) {
     case 8: launch_att_lpt<8, 1>(c, a, grid, lds); break;
@@ -1641,11 +1180,11 @@ static int trace_kpt(int zs) { return zs <= 256 ? 1 : zs <= 512 ? 2 : zs <= 1024
 
 static void launch_trace(moc_ctx *c, const TraceArgs &ta, bool write)
 {
-    const long long nst = c->nt2 * c->np;
+    const long long nst = ta.s_end - ta.s_begin;
     const int kpt = trace_kpt(c->zs);
     int nt = (c->zs + kpt - 1) / kpt;
     nt = ((nt + 63) / 64) * 64;
-    const int grid = (int)(nst < (1 << 20) ? nst : (1 << 20));
+    const int grid = (int)(nst < (1 << 20) ? (nst < 1 ? 1 : nst) : (1 << 20));
 #define MOC_TRACE_LAUNCH(K)                                                                     \
     do {                                                                                       \
         if (write)                                                                             \
@@ -1675,11 +1214,13 @@ static int materialize_psi(moc_ctx *c)
     return MOC_OK;
 }
 
-static int ensure_records(moc_ctx *c, long long S)
+static int ensure_records(moc_ctx *c, long long S, long long budget)
 {
     if (S <= c->rec_cap)
         return MOC_OK;
     long long cap = S + S / 8 + 1024;
+    if (cap > budget)
+        cap = budget > S ? budget : S;
     if (c->d_rec) {
         HIP_TRY(hipFree(c->d_rec));
         c->d_rec = nullptr;
@@ -1703,14 +1244,39 @@ static int prep_coefficients(moc_ctx *c, double *acc)
     return MOC_OK;
 }
 
-// Axial trace of the next sweep into the piece records: count pass,
-// offsets, then the write pass.  With commit the final z heights are
-// stored (a sweep); without, the z state is left as it is (the adjoint).
-static int trace_records(moc_ctx *c, bool commit, long long *S_out)
+// A run of whole z-stacks whose piece records are written and attenuated
+// together; rec[0] holds record (and, for explicit draws, serial) index
+// rec_base.
+struct Chunk {
+    long long s0, s1, rec_base, S;
+};
+
+// Records one chunk may hold: the caller's max_records, else what device
+// memory allows next to everything else (1 GiB kept free), else all of S.
+static int records_budget(moc_ctx *c, long long S, long long *budget)
+{
+    if (c->opt.max_records > 0) {
+        *budget = c->opt.max_records;
+        return MOC_OK;
+    }
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    long long avail = ((long long)free_b + c->rec_cap * (long long)sizeof(float4) - (1LL << 30)) /
+                      (long long)sizeof(float4);
+    const long long want = S + S / 8 + 1024;
+    *budget = avail >= want ? want : avail;
+    if (*budget < 1)
+        return moc_set_error(MOC_ERR_NOMEM, "no device memory left for piece records");
+    return MOC_OK;
+}
+
+// Count pass and offsets of the next sweep (z unchanged), then the chunks:
+// one when all records fit, else runs of whole stacks of at most the
+// budget's records each.
+static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out)
 {
     const long long nst = c->nt2 * c->np;
     TraceArgs ta = trace_args(c);
-    ta.commit = commit ? 1 : 0;
     launch_trace(c, ta, false);
     HIP_TRY(hipGetLastError());
     size_t tb = c->scan_tmp_bytes;
@@ -1719,52 +1285,104 @@ static int trace_records(moc_ctx *c, bool commit, long long *S_out)
     long long S = 0;
     HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    TRY(ensure_records(c, S));
-    ta.rec = c->d_rec;
-    ta.rec_cap = c->rec_cap;
-    if (c->opt.qsr_mode == MOC_QSR_LIBC) {
-        // region draws from the host's libc rand() in the reference's serial
-        // order (src/solver.c:481): stack-major, then the trace order
-        if (S > c->stream_cap) {
-            if (c->d_stream)
-                HIP_TRY(hipFree(c->d_stream));
-            c->d_stream = nullptr;
-            c->stream_cap = 0;
-            TRY(dalloc(&c->d_stream, S + 1));
-            c->stream_cap = S + 1;
+    long long budget = 0;
+    TRY(records_budget(c, S, &budget));
+    chunks.clear();
+    if (S <= budget) {
+        chunks.push_back({0, nst, 0, S});
+        TRY(ensure_records(c, S, budget));
+    } else {
+        std::vector<long long> sc(nst);
+        HIP_TRY(hipMemcpy(sc.data(), c->d_stack_cnt, nst * 8, hipMemcpyDeviceToHost));
+        long long base = 0, biggest = 0;
+        long long s0 = 0, acc = 0;
+        for (long long s = 0; s < nst; s++) {
+            if (acc > 0 && acc + sc[s] > budget) {
+                chunks.push_back({s0, s, base, acc});
+                biggest = std::max(biggest, acc);
+                base += acc;
+                s0 = s;
+                acc = 0;
+            }
+            acc += sc[s];
         }
-        std::vector<int32_t> draws((size_t)S);
-        for (long long q = 0; q < S; q++)
-            draws[q] = (int32_t)(rand() % c->nsr);
-        HIP_TRY(hipMemcpyAsync(c->d_stream, draws.data(), S * 4, hipMemcpyHostToDevice, c->stream));
+        chunks.push_back({s0, nst, base, acc});
+        biggest = std::max(biggest, acc);
+        if (biggest > budget && c->opt.max_records == 0)
+            return moc_set_error(MOC_ERR_NOMEM, "one z-stack needs %lld piece records, more than "
+                                 "device memory holds", biggest);
+        // exact fit: chunks are reused, never grown
+        if (biggest > c->rec_cap) {
+            if (c->d_rec)
+                HIP_TRY(hipFree(c->d_rec));
+            c->d_rec = nullptr;
+            c->rec_cap = 0;
+            TRY(dalloc(&c->d_rec, biggest));
+            c->rec_cap = biggest;
+        }
+    }
+    if (c->opt.qsr_mode == MOC_QSR_LIBC) {
         size_t tb2 = c->scan_tmp_bytes;
         HIP_TRY(rocprim::exclusive_scan(c->d_scan_tmp, tb2, c->d_stack_cnt, c->d_stack_off,
                                         (long long)0, (size_t)nst, rocprim::plus<long long>(),
                                         c->stream));
-        ta.qsr_stream = c->d_stream;
-        HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    launch_trace(c, ta, true);
-    HIP_TRY(hipGetLastError());
     *S_out = S;
     return MOC_OK;
 }
 
-extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+// Write pass of one chunk (z committed with commit).  With the host's
+// rand() draws, the chunk's draws come next in the serial order
+// (src/solver.c:481: stack-major, then the trace order), which is the
+// chunk's record range.
+static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit)
 {
-    if (!c)
-        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
-    HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    // (1) coefficients + accumulator
-    TRY(prep_coefficients(c, c->d_acc));
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    // (2)-(4) trace: count, offsets, records; z heights committed
-    long long S = 0;
-    TRY(trace_records(c, true, &S));
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    // (5) attenuation
+    TraceArgs ta = trace_args(c);
+    ta.commit = commit ? 1 : 0;
+    ta.rec = c->d_rec;
+    ta.rec_cap = c->rec_cap;
+    ta.rec_base = ch.rec_base;
+    ta.s_begin = ch.s0;
+    ta.s_end = ch.s1;
+    if (c->opt.qsr_mode == MOC_QSR_LIBC) {
+        if (ch.S > c->stream_cap) {
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (c->d_stream)
+                HIP_TRY(hipFree(c->d_stream));
+            c->d_stream = nullptr;
+            c->stream_cap = 0;
+            TRY(dalloc(&c->d_stream, ch.S + 1));
+            c->stream_cap = ch.S + 1;
+        }
+        std::vector<int32_t> draws((size_t)ch.S);
+        for (long long q = 0; q < ch.S; q++)
+            draws[q] = (int32_t)(rand() % c->nsr);
+        // the previous chunk's write pass has read the buffer before this copy
+        HIP_TRY(hipMemcpyAsync(c->d_stream, draws.data(), ch.S * 4, hipMemcpyHostToDevice, c->stream));
+        ta.qsr_stream = c->d_stream;
+        launch_trace(c, ta, true);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));   // draws stays alive until here
+        return MOC_OK;
+    }
+    launch_trace(c, ta, true);
+    HIP_TRY(hipGetLastError());
+    return MOC_OK;
+}
+
+// what the record buffer holds after write_chunk
+static void note_chunk(moc_ctx *c, const Chunk &ch)
+{
+    c->rec_base = ch.rec_base;
+    c->rec_t_begin = ch.s0 * c->zs;
+    c->rec_t_end = ch.s1 * c->zs;
+}
+
+// the attenuation arguments of the sweep, for the tracks of one chunk
+static AttArgs att_args(moc_ctx *c, const Chunk &ch)
+{
     AttArgs aa;
+    memset(&aa, 0, sizeof(aa));
     aa.off = c->d_off;
     aa.rec = c->d_rec;
     aa.coef = c->d_coef;
@@ -1784,9 +1402,51 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     aa.dx = c->table_dx;
     aa.hdx = 0.5f * c->table_dx;
     aa.maxv = c->table_maxv;
-    launch_attenuate(c, aa);
-    HIP_TRY(hipGetLastError());
+    aa.rec_base = ch.rec_base;
+    aa.t_begin = ch.s0 * c->zs;
+    aa.t_end = ch.s1 * c->zs;
+    return aa;
+}
+
+extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    // (1) coefficients + accumulator
+    TRY(prep_coefficients(c, c->d_acc));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    // (2)-(5) trace: count, offsets; then per chunk of stacks the records
+    // (z heights committed) and their attenuation
+    long long S = 0;
+    std::vector<Chunk> chunks;
+    TRY(plan_trace(c, chunks, &S));
+    float ms_trace = 0.f, ms_att = 0.f;
+    {
+        float ms = 0.f;
+        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+        ms_trace += ms;
+    }
+    for (size_t k = 0; k < chunks.size(); k++) {
+        HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+        TRY(write_chunk(c, chunks[k], true));
+        note_chunk(c, chunks[k]);
+        HIP_TRY(hipEventRecord(c->ev[6], c->stream));
+        launch_attenuate(c, att_args(c, chunks[k]));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[7], c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev[7]));
+        float a = 0.f, b = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
+        HIP_TRY(hipEventElapsedTime(&b, c->ev[6], c->ev[7]));
+        ms_trace += a;
+        ms_att += b;
+    }
     c->psi_pending = false;
+    c->chunks = (long)chunks.size();
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     // (6) flux = (float)accumulator
     if (c->opt.tally == MOC_TALLY_F64) {
@@ -1797,14 +1457,16 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     HIP_TRY(hipEventRecord(c->ev[4], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[4]));
     TRY(check_err(c));
-    float ms[4];
-    for (int q = 0; q < 4; q++)
-        HIP_TRY(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
-    c->timing.prep_ms = ms[0];
-    c->timing.trace_ms = ms[1];
-    c->timing.attenuate_ms = ms[2];
-    c->timing.finalize_ms = ms[3];
-    c->timing.sweep_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
+    float ms_prep = 0.f, ms_fin = 0.f, ms_all = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_fin, c->ev[3], c->ev[4]));
+    HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[4]));
+    c->timing.prep_ms = ms_prep;
+    c->timing.trace_ms = ms_trace;
+    c->timing.attenuate_ms = ms_att;
+    c->timing.finalize_ms = ms_fin;
+    c->timing.sweep_ms = ms_all;
+    c->timing.chunks = c->chunks;
     c->last_segments = (long)S;
     c->timing.segments = (long)S;
     c->sweeps++;
@@ -2041,9 +1703,13 @@ extern "C" int moc_get_track_records(moc_ctx *c, long track, float *ds, float *z
         return moc_set_error(MOC_ERR_INVALID, "moc_get_track_records: bad track");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (track < c->rec_t_begin || track >= c->rec_t_end)
+        return moc_set_error(MOC_ERR_STATE, "moc_get_track_records: the last sweep ran in record "
+                             "chunks and kept only tracks [%lld, %lld)", c->rec_t_begin, c->rec_t_end);
     long long o[2];
     HIP_TRY(hipMemcpy(o, c->d_off + track, 16, hipMemcpyDeviceToHost));
     long long cnt = o[1] - o[0];
+    o[0] -= c->rec_base;
     if (n)
         *n = (long)cnt;
     long long m = cnt < cap ? cnt : cap;
@@ -2183,7 +1849,8 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     TRY(prep_coefficients(c, nullptr));
     long long S = 0;
-    TRY(trace_records(c, false, &S));
+    std::vector<Chunk> chunks;
+    TRY(plan_trace(c, chunks, &S));
     if (!c->d_adj_w) {
         TRY(dalloc(&c->d_adj_w, (size_t)c->nval));
         TRY(dalloc(&c->d_adj_gs, (size_t)c->nsr * c->G));
@@ -2226,7 +1893,6 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nval * 4, 256)), dim3(256), 0, c->stream,
                        c->d_adj_acc, c->nval * 4);
     hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, c->stream, c->d_adj_J, 1LL);
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     AdjArgs aa;
     aa.off = c->d_off;
     aa.rec = c->d_rec;
@@ -2251,6 +1917,15 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     aa.hdx = 0.5f * c->table_dx;
     aa.maxv = c->table_maxv;
     const size_t lds = (size_t)c->table_N * sizeof(float2);
+    float ms_k = 0.f;
+    for (const Chunk &ch : chunks) {
+        TRY(write_chunk(c, ch, false));
+        note_chunk(c, ch);
+        aa.rec = c->d_rec;
+        aa.rec_base = ch.rec_base;
+        aa.t_begin = ch.s0 * c->zs;
+        aa.t_end = ch.s1 * c->zs;
+        HIP_TRY(hipEventRecord(c->ev[5], c->stream));
 #define MOC_ADJ(L)                                   \
     do {                                             \
         if (c->quad)                                 \
@@ -2258,23 +1933,28 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         else                                         \
             launch_adj_lpt<L, false>(c, aa, (int)grid, lds); \
     } while (0)
-    switch (c->lpt) {
-    case 8: MOC_ADJ(8); break;
-    case 16: MOC_ADJ(16); break;
-    case 32: MOC_ADJ(32); break;
-    default: MOC_ADJ(64); break;
-    }
+        switch (c->lpt) {
+        case 8: MOC_ADJ(8); break;
+        case 16: MOC_ADJ(16); break;
+        case 32: MOC_ADJ(32); break;
+        default: MOC_ADJ(64); break;
+        }
 #undef MOC_ADJ
-    HIP_TRY(hipGetLastError());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[6], c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev[6]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, c->ev[5], c->ev[6]));
+        ms_k += t;
+    }
     hipLaunchKernelGGL(k_adj_finish, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
                        c->d_adj_acc, c->d_adj_gc, c->d_adj_gs, (long long)c->nsr, c->G, c->fai);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[2]));
     TRY(check_err(c));
-    float ms_all = 0, ms_k = 0;
+    float ms_all = 0;
     HIP_TRY(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
-    HIP_TRY(hipEventElapsedTime(&ms_k, c->ev[1], c->ev[2]));
     c->timing.adjoint_ms = ms_all;
     c->timing.adjoint_kernel_ms = ms_k;
     if (J)

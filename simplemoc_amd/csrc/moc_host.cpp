@@ -516,3 +516,46 @@ extern "C" int moc_problem_generate(const moc_input *in, const moc_gen_params *g
     *out = p;
     return MOC_OK;
 }
+
+// Piece bound of the axial trace (src/solver.c:409-525).  Within one 2D
+// segment every piece that does not end the segment moves the current
+// plane index one plane on, and the loop ends when that plane reaches the
+// domain boundary: at most cai*fai + 2 pieces, whatever z was (this covers
+// the last segment, whose z resets after every piece while the plane index
+// keeps moving).  Before the last segment z moves continuously, so a
+// segment of 2D length L also has at most floor(L |cot theta| / dz) + 2
+// pieces.  Both get a margin for rounding.
+extern "C" int moc_record_bounds(const moc_problem *p, long long *per_stack, long long *total)
+{
+    if (!p || !p->in.derived)
+        return moc_set_error(MOC_ERR_INVALID, "moc_record_bounds: problem missing or not derived");
+    const moc_input &I = p->in;
+    const long long nt2 = I.ntracks_2D;
+    const int np = I.n_polar_angles, zs = I.z_stacked;
+    const double node_dz = (double)(I.height / (float)I.decomp_assemblies_ax);
+    const double fine_dz = node_dz / (double)(I.cai * I.fai);
+    const long long pmax = (long long)I.cai * I.fai + 4;
+    long long sum = 0;
+    for (long long i = 0; i < nt2; i++) {
+        const long long off = p->seg_offset[i], nseg = p->seg_offset[i + 1] - off;
+        for (int j = 0; j < np; j++) {
+            const double th = (double)p->polar_angle[j];
+            const double cot = fabs(cos(th) / sin(th));
+            long long b = 0;
+            if (nseg > 0) {
+                for (long long n = 0; n + 1 < nseg; n++) {
+                    const double planes = (double)p->seg_length[off + n] * cot / fine_dz;
+                    const long long c = planes < (double)pmax ? (long long)planes + 4 : pmax;
+                    b += c < pmax ? c : pmax;
+                }
+                b += pmax;
+            }
+            if (per_stack)
+                per_stack[i * np + j] = b;
+            sum += b * zs;
+        }
+    }
+    if (total)
+        *total = sum;
+    return MOC_OK;
+}

@@ -51,6 +51,10 @@ struct TraceArgs {
     float sep;                  // axial_z_sep
     uint64_t sweep_key;         // hash key of (seed, sweep)
     int commit;                 // write pass stores the final z heights
+    uint64_t seed;              // batched sweeps: the key is hashed on the device
+    const long long *sweep_ctr; // from (seed, *sweep_ctr)
+    long long rec_base;         // record (and serial draw) index of rec[0]
+    long long s_begin, s_end;   // stacks of this launch
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.
@@ -74,6 +78,8 @@ struct AttArgs {
     float dx;
     float hdx;                  // 0.5f * dx
     float maxv;
+    long long rec_base;         // record index of rec[0]
+    long long t_begin, t_end;   // tracks of this launch
 };
 
 }  // namespace moc

@@ -1,0 +1,536 @@
+// moc_sweep.h -- device code of one transport sweep, shared by the
+// single-problem context (moc_device.hip) and the batched one
+// (moc_batch.hip).  Only __device__ functions live here; each translation
+// unit wraps them in its own kernels.
+//
+//   fit_coefficients  per (FSR, group): quadratic-fit coefficients and
+//                     sigma_t packed in one float4 (src/solver.c:55-154)
+//   trace_stack       per z-stack: axial ray trace (src/solver.c:366-527),
+//                     piece counts or piece records {ds, zin, fsr}
+//   attenuate_track   per 3D track: groups across lanes, psi in registers,
+//                     one atomic per (piece, group) (src/solver.c:14-280,
+//                     :1040-1138)
+//
+// Arithmetic that decides control flow or table cells (trace, table index)
+// is bit-identical to the reference's: IEEE division and conversions in
+// the same float/double types, no contraction (-ffp-contract=off).
+#ifndef MOC_SWEEP_H
+#define MOC_SWEEP_H
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/simplemoc.h"
+#include "moc_kernels.h"
+
+namespace moc {
+
+// ---------------------------------------------------------------------
+// hashing of region draws (same definition as the oracle's qsr hash)
+__host__ __device__ static inline uint64_t mix64(uint64_t z)
+{
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ULL;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return z;
+}
+__host__ __device__ static inline uint64_t sweep_key(uint64_t seed, long long sweep)
+{
+    return mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sweep + 1));
+}
+__device__ static inline uint64_t track_key(uint64_t skey, long long t)
+{
+    return mix64(skey ^ ((uint64_t)t * 0xD1B54A32D192ED03ULL));
+}
+__device__ static inline uint32_t piece_region(uint64_t tkey, long long n, int p, long long nsr)
+{
+    uint64_t h = mix64(tkey + (((uint64_t)n << 32) | (uint64_t)(uint32_t)p));
+    return (uint32_t)(((h >> 32) * (uint64_t)nsr) >> 32);
+}
+
+// ---------------------------------------------------------------------
+// (1) source-fit coefficients of one (FSR, group) element e = fsr*G + g:
+// {c0, c1, c2, sigma_t} for the quadratic source, {source, 0, 0, sigma_t}
+// for the flat one.  Edge rule of src/solver.c:55-138 with row indices
+// clamped into [0, fai-1] (a no-op for fai >= 3; defined fallback for
+// fai < 3).  dz is the fine interval height.
+__device__ __forceinline__ float4 fit_coefficients(const float *__restrict__ src,
+                                                   const float *__restrict__ sig, long long e,
+                                                   int G, int fai, int quad, float dz)
+{
+    const float two_dz = 2.f * dz;
+    const float two_dz2 = 2.f * dz * dz;
+    const long long fsr = e / G;
+    const int g = (int)(e - fsr * G);
+    const long long r = fsr / fai;
+    const int fine = (int)(fsr - r * fai);
+    float4 c;
+    if (quad) {
+        int r1, r2, r3;
+        if (fine == 0) {
+            r1 = 0; r2 = 1; r3 = 2;
+        } else if (fine == fai - 1) {
+            r1 = fine - 2; r2 = fine - 1; r3 = fine;
+        } else {
+            r1 = fine - 1; r2 = fine; r3 = fine + 1;
+        }
+        r1 = r1 < 0 ? 0 : (r1 > fai - 1 ? fai - 1 : r1);
+        r2 = r2 < 0 ? 0 : (r2 > fai - 1 ? fai - 1 : r2);
+        r3 = r3 > fai - 1 ? fai - 1 : r3;
+        const float *base = src + r * fai * G + g;
+        const float y1 = base[(long long)r1 * G];
+        const float y2 = base[(long long)r2 * G];
+        const float y3 = base[(long long)r3 * G];
+        c.x = y2;
+        c.y = (y1 - y3) / two_dz;
+        c.z = (y1 - 2.f * y2 + y3) / two_dz2;
+    } else {
+        c.x = src[e];
+        c.y = 0.f;
+        c.z = 0.f;
+    }
+    c.w = sig[r * G + g];
+    return c;
+}
+
+// ---------------------------------------------------------------------
+// (2) axial ray trace (src/solver.c:366-527).  One workgroup per
+// (2D track, polar angle) z-stack; thread `tid` owns the stacked tracks
+// k = tid*KPT .. tid*KPT+KPT-1 with their z heights in registers.
+//
+// The reference walks k serially inside each 2D segment n and changes the
+// loop bounds while it runs: going up, every exit does end-- before the
+// next k is tested; going down, begin++ only affects later segments.  Here
+// every active track first traces segment n tentatively (pieces, exit,
+// final z).  Going up, track k is processed iff k < end - E(k), with E(k)
+// the exits of the lower tracks; that test fails monotonically, so the
+// processed tracks are a prefix [begin, kstop) found with one block scan
+// and one block minimum.  Processed tracks then commit (and, in the write
+// pass, re-trace to emit records); the others keep their z.  The result
+// equals the serial loop, including the quirk of retiring the wrong track
+// when several exit in one segment.
+struct SegResult {
+    int pieces;
+    int exited;
+    float zfin;
+};
+
+struct EmitCtx {
+    float4 *rec;
+    long long rec_cap;
+    long long base;             // record index of the track's next piece
+    const int32_t *stream;      // explicit region draws (or null)
+    long long serial;           // serial index of the first piece (stream mode)
+    uint64_t tkey;
+    long long nsr;
+    int fai;
+    int quad;
+    float dz_att;
+    unsigned *err;
+};
+
+// One (track, 2D segment): the while loop of src/solver.c:409-525.
+template <bool EMIT>
+__device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k, bool pos,
+                                                   bool last_seg, long long n, const TraceArgs &a,
+                                                   double c, EmitCtx *ec)
+{
+    float sr = s_full;
+    int curr = pos ? (int)(zk / a.dzf) : (int)ceil((double)(zk / a.dzf));
+    bool complete = false;
+    int p = 0;
+    int exited = 0;
+    while (!complete) {
+        bool reset = false;
+        float ds;
+        float zn = (float)((double)zk + (double)sr * c);
+        int ni = pos ? (int)(zn / a.dzf) : (int)ceil((double)(zn / a.dzf));
+        if (ni == curr) {
+            complete = true;
+            ds = sr;
+        } else {
+            curr += pos ? 1 : -1;
+            zn = (float)(a.fine_dz * (double)(float)curr);
+            ds = (float)((double)(zn - zk) / c);
+            sr -= ds;
+            if (sr <= 0.f)
+                complete = true;
+            if (zn <= 0.f || (double)zn >= a.node_dz) {
+                complete = true;
+                exited = 1;
+                reset = true;
+            }
+        }
+        if (EMIT) {
+            uint32_t qsr;
+            if (ec->stream) {
+                int32_t v = ec->stream[ec->serial + p];
+                if (v < 0 || (long long)v >= ec->nsr) {
+                    atomicOr(ec->err, ERR_QSR_RANGE);
+                    v = 0;
+                }
+                qsr = (uint32_t)v;
+            } else {
+                qsr = piece_region(ec->tkey, n, p, ec->nsr);
+            }
+            // cell of the piece start, as attenuate_fluxes computes it (:38-45)
+            const int cell = (int)(zk / ec->dz_att);
+            float zin = zk - ec->dz_att * ((float)cell + 0.5f);
+            int fine = cell % ec->fai;
+            if (ec->quad) {
+                if (fine == 0)
+                    zin -= ec->dz_att;
+                else if (fine == ec->fai - 1)
+                    zin += ec->dz_att;
+            }
+            if (fine < 0) {
+                atomicOr(ec->err, ERR_FINE_RANGE);
+                fine = 0;
+            }
+            const long long q = ec->base + p;
+            if (q < ec->rec_cap)
+                ec->rec[q] = make_float4(ds, zin, __uint_as_float(qsr * (uint32_t)ec->fai + (uint32_t)fine), 0.f);
+            else
+                atomicOr(ec->err, ERR_RECORD_CAP);
+        }
+        p++;
+        if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
+            atomicOr(a.err, ERR_TRACE_RUNAWAY);
+            complete = true;
+        }
+        if (last_seg || reset)
+            zk = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
+        else
+            zk = zn;
+    }
+    SegResult r;
+    r.pieces = p;
+    r.exited = exited;
+    r.zfin = zk;
+    return r;
+}
+
+// block-wide helpers (blockDim.x a multiple of 64, at most 1024)
+__device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        s_w[w] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int q = 0; q < nw; q++) {
+        int t = s_w[q];
+        pre += (q < w) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+__device__ __forceinline__ int block_min(int v, int *s_w)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    if (lane == 0)
+        s_w[w] = v;
+    __syncthreads();
+    int m = s_w[0];
+    for (int q = 1; q < nw; q++)
+        m = min(m, s_w[q]);
+    __syncthreads();
+    return m;
+}
+
+// One z-stack s of the problem described by `a` (every thread of the
+// block calls it with the same s; s_w is 16 ints of LDS).
+template <bool WRITE, int KPT>
+__device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uint64_t skey, int *s_w)
+{
+    const int zs = a.zs;
+    const int k0 = threadIdx.x * KPT;
+    {
+        const long long i = s / a.np;
+        const int j = (int)(s - i * a.np);
+        const long long t0 = s * zs;
+        const bool pos = j < a.np / 2;
+        const double c = a.cosd[j];
+        const double sn = a.sind[j];
+        float z[KPT];
+        int cnt[KPT];
+        long long base[KPT];
+        uint64_t tkey[KPT];
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = k0 + r;
+            z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
+            cnt[r] = 0;
+            base[r] = (WRITE && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
+            tkey[r] = (WRITE && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
+        }
+        int begin = 0, end = zs;
+        long long serial = (WRITE && a.qsr_stream) ? a.stack_off[s] - a.rec_base : 0;
+        const long long off = a.seg_off[i];
+        const long long nseg = a.seg_off[i + 1] - off;
+        for (long long n = 0; n < nseg; n++) {
+            const float s_full = (float)((double)a.seg_len[off + n] / sn);
+            const bool last = (n == nseg - 1);
+            SegResult res[KPT];
+            bool act[KPT];
+            int nx = 0;
+#pragma unroll
+            for (int r = 0; r < KPT; r++) {
+                const int k = k0 + r;
+                act[r] = (k >= begin && k < end);
+                if (act[r]) {
+                    res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
+                } else {
+                    res[r].pieces = 0;
+                    res[r].exited = 0;
+                    res[r].zfin = z[r];
+                }
+                nx += res[r].exited;
+            }
+            // which active tracks the serial loop reaches in this segment
+            bool proc[KPT];
+            int ex_total;
+            int ex_pre = block_excl_scan(nx, s_w, &ex_total);
+            if (pos) {
+                int fail = INT_MAX;
+                int e = ex_pre;
+#pragma unroll
+                for (int r = 0; r < KPT; r++) {
+                    const int k = k0 + r;
+                    if (act[r] && k >= end - e && fail == INT_MAX)
+                        fail = k;
+                    e += res[r].exited;
+                }
+                const int kstop = (ex_total == 0) ? INT_MAX : block_min(fail, s_w);
+                int nproc_exits = 0;
+#pragma unroll
+                for (int r = 0; r < KPT; r++) {
+                    proc[r] = act[r] && (k0 + r) < kstop;
+                    nproc_exits += proc[r] ? res[r].exited : 0;
+                }
+                if (kstop != INT_MAX) {
+                    int tot;
+                    (void)block_excl_scan(nproc_exits, s_w, &tot);
+                    end -= tot;
+                } else {
+                    end -= ex_total;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < KPT; r++)
+                    proc[r] = act[r];
+                begin += ex_total;
+            }
+            // serial index of each track's first piece (region stream)
+            long long ser = 0;
+            if (WRITE && a.qsr_stream) {
+                int np_loc = 0;
+#pragma unroll
+                for (int r = 0; r < KPT; r++)
+                    np_loc += proc[r] ? res[r].pieces : 0;
+                int ptot;
+                int ppre = block_excl_scan(np_loc, s_w, &ptot);
+                ser = serial + ppre;
+                serial += ptot;
+            }
+#pragma unroll
+            for (int r = 0; r < KPT; r++) {
+                if (!proc[r])
+                    continue;
+                if (WRITE) {
+                    EmitCtx ec;
+                    ec.rec = a.rec;
+                    ec.rec_cap = a.rec_cap;
+                    ec.base = base[r] + cnt[r];
+                    ec.stream = a.qsr_stream;
+                    ec.serial = ser;
+                    ec.tkey = tkey[r];
+                    ec.nsr = a.nsr;
+                    ec.fai = a.fai;
+                    ec.quad = a.quad;
+                    ec.dz_att = a.dz_att;
+                    ec.err = a.err;
+                    (void)trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
+                    ser += res[r].pieces;
+                }
+                cnt[r] += res[r].pieces;
+                z[r] = res[r].zfin;
+            }
+        }
+        long long stack_total = 0;
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = k0 + r;
+            if (k < zs) {
+                if (WRITE && a.commit)
+                    a.z[t0 + k] = z[r];
+                else
+                    a.cnt[t0 + k] = cnt[r];
+            }
+            stack_total += cnt[r];
+        }
+        if (!WRITE && a.stack_cnt) {
+            int tot;
+            (void)block_excl_scan((int)stack_total, s_w, &tot);
+            if (threadIdx.x == 0)
+                a.stack_cnt[s] = tot;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------
+// (3) attenuation (src/solver.c:14-280 quadratic, :1040-1138 flat)
+// 1 - exp(-tau): the reference table (src/solver.c:1441-1464) or -expm1f.
+// The index clamp is branch-free: v <= 0 (and NaN) -> cell 0, which is the
+// reference's truncation for -1 < v < 0; v >= N-1 -> cell N-1 (never
+// reached below maxVal).
+template <int EXPM>
+__device__ __forceinline__ float exp_term(const float2 *tab, float tau, float dx, float hdx,
+                                          float maxv, int N)
+{
+    if (EXPM != MOC_EXP_TABLE)
+        return -expm1f(-tau);
+    const float v = tau / dx + hdx;
+    const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
+    const float2 sl = tab[iv];
+    const float F = sl.x * tau + sl.y;
+    return tau > maxv ? 1.0f : F;
+}
+
+// broadcast lane `src` of x to the lanes of its track (LPT lanes per track)
+template <int LPT>
+__device__ __forceinline__ float bcast(float x, int src_in_track, int lane_base)
+{
+    if (LPT == 64)
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src_in_track));
+    return __shfl(x, lane_base + src_in_track, 64);
+}
+
+// One track per LPT lanes; lane l owns groups g = l + r*LPT (r < GPL).
+// Records are read LPT at a time (one coalesced load) and broadcast; the
+// coefficients of PB pieces are requested together so their latency, and
+// the drain of earlier atomics the compiler waits for, is paid once per PB
+// pieces.
+// Track t of the problem described by `a`, on the LPT lanes l = 0..LPT-1
+// starting at wave lane lane_base (all of them call it with the same t).
+// s_tab holds the problem's exponential table.
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
+__device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *s_tab, long long t,
+                                                int l, int lane_base)
+{
+    const int G = a.G;
+    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
+    const int N = a.N;
+    const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
+    int gcl[GPL];
+#pragma unroll
+    for (int r = 0; r < GPL; r++)
+        gcl[r] = min(l + r * LPT, G - 1);   // clamped group for unconditional loads
+    {
+        const int j = (int)((t / a.zs) % a.np);
+        const long long i = t / ((long long)a.zs * a.np);
+        const float mu = a.mu[j];
+        const float mu2 = mu * mu;
+        const float azw = a.az[i];
+        const float weight = QUAD ? a.pw[t] * azw : a.pw[t] * azw * mu;
+        float psi[GPL];
+#pragma unroll
+        for (int r = 0; r < GPL; r++)
+            // a pending renormalisation (src/solver.c:1224) is applied here
+            psi[r] = a.psi[t * G + gcl[r]] * pscale;
+        const long long q0 = a.off[t] - a.rec_base, q1 = a.off[t + 1] - a.rec_base;
+        for (long long qc = q0; qc < q1; qc += LPT) {
+            const int nq = (int)min((long long)LPT, q1 - qc);
+            const float4 mine = a.rec[qc + min(l, nq - 1)];
+            for (int p = 0; p < nq; p += PB) {
+                float ds[PB], zin[PB];
+                uint32_t fsr[PB];
+                float4 cf[PB][GPL];
+#pragma unroll
+                for (int u = 0; u < PB; u++) {
+                    const int pu = min(p + u, nq - 1);
+                    ds[u] = bcast<LPT>(mine.x, pu, lane_base);
+                    zin[u] = bcast<LPT>(mine.y, pu, lane_base);
+                    fsr[u] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
+#pragma unroll
+                    for (int r = 0; r < GPL; r++)
+                        cf[u][r] = a.coef[(size_t)fsr[u] * G + gcl[r]];
+                }
+#pragma unroll
+                for (int u = 0; u < PB; u++) {
+                    if (p + u >= nq)
+                        break;
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        const int g = l + r * LPT;
+                        if (g >= G)
+                            continue;
+                        const float4 c = cf[u][r];
+                        const float sig = c.w;
+                        const float tau = sig * ds[u];
+                        const float F = exp_term<EXPM>(s_tab, tau, dx, hdx, maxv, N);
+                        const float ps = psi[r];
+                        float tally;
+                        if (QUAD) {
+                            const float zi = zin[u];
+                            const float q0v = c.x + c.y * zi + c.z * zi * zi;
+                            const float q1v = c.y + 2.f * c.z * zi;
+                            const float q2v = c.z;
+                            const float sig2 = sig * sig;
+                            const float reuse = tau * (tau - 2.f) + 2.f * F / (sig * sig2);
+                            const float fi = (q0v * tau + (sig * ps - q0v) * F) / sig2 +
+                                             q1v * mu * reuse +
+                                             q2v * mu2 * (tau * (tau * (tau - 3.f) + 6.f) - 6.f * F) /
+                                                 (3.f * sig2 * sig2);
+                            tally = weight * fi;
+                            const float t1 = q0v * F / sig;
+                            const float t2 = q1v * mu * (tau - F) / sig2;
+                            const float t3 = q2v * mu2 * reuse;
+                            const float t4 = ps * (1.f - F);
+                            psi[r] = t1 + t2 + t3 + t4;
+                        } else {
+                            const float qv = c.x / sig;
+                            const float dpsi = (ps - qv) * F;
+                            tally = weight * dpsi;
+                            psi[r] = ps - dpsi;
+                        }
+                        if (T64)
+                            atomicAdd(a.acc64 + (size_t)fsr[u] * G + g, (double)tally);
+                        else
+                            atomicAdd(a.flux32 + (size_t)fsr[u] * G + g, tally);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < GPL; r++) {
+            const int g = l + r * LPT;
+            if (g < G)
+                a.psi[t * G + g] = psi[r];
+        }
+    }
+}
+
+
+}  // namespace moc
+
+#endif

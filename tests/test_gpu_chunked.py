@@ -1,0 +1,89 @@
+"""Sweeps whose piece records do not fit at once run in chunks of whole
+z-stacks (count pass and offsets for all stacks, then per chunk the
+record write and its attenuation).  Forcing small record budgets
+(Options.max_records) on problems that fit must not change anything:
+* hash draws: z heights, piece counts and angular flux identical to the
+  single-chunk sweep, scalar flux within 1e-6 (FP64 tally, only the order
+  of the atomic additions differs), over several iterations;
+* the reference's rand() draws, consumed chunk by chunk in serial order:
+  the reference's own golden run (piece counts, z heights, k_eff);
+* the adjoint: J and gradients within 1e-9 of the single-chunk adjoint.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import simplemoc_amd as sm
+from helpers import GOLDEN, make_input
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return np.abs(x - ref).max() / max(np.abs(ref).max(), 1e-30)
+
+
+def problem():
+    inp = make_input(radial_ray_sep=0.3, n_egroups=16, n_azimuthal=8, height=10.0, cai=3)
+    return sm.Problem.generate(inp, seed=5)
+
+
+@pytest.mark.parametrize("split", [3, 17])
+def test_chunked_sweep_matches_single_chunk(split):
+    prob = problem()
+    with sm.Solver(prob, sm.Options(tally="f64")) as s:
+        out1 = s.iterate(3)
+        st1 = s.state()
+        S = out1[-1][0]
+    with sm.Solver(prob, sm.Options(tally="f64", max_records=S // split + 1)) as s:
+        out2 = s.iterate(3)
+        st2 = s.state()
+        chunks = s.timing()["chunks"]
+    print(f"split {split}: {chunks} chunks, {S} pieces")
+    assert chunks >= split
+    assert [x[0] for x in out1] == [x[0] for x in out2]
+    np.testing.assert_array_equal(st1["z_height"], st2["z_height"])
+    np.testing.assert_array_equal(st1["f_psi"], st2["f_psi"])
+    assert rel(st2["fine_flux"], st1["fine_flux"]) <= 1e-6
+    for (_, k1, _), (_, k2, _) in zip(out1, out2):
+        assert abs(k1 - k2) <= 1e-6 * max(1.0, abs(k1))
+
+
+def test_chunked_libc_draws_match_reference_golden(golden_index, libc):
+    meta = golden_index["tiny_quad"]
+    g = np.load(os.path.join(GOLDEN, "tiny_quad.npz"))
+    inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference")
+    S0 = int(g["segments"][0])
+    with sm.Solver(prob, sm.Options(qsr="libc", max_records=S0 // 7 + 1)) as s:
+        libc.srand(meta["sweep_seed"])
+        out = s.iterate(meta["iterations"])
+        st = s.state()
+        assert s.timing()["chunks"] >= 7
+    assert [x[0] for x in out] == g["segments"].tolist()
+    np.testing.assert_array_equal(st["z_height"], g["z_height"])
+    for (_, k, _), kr in zip(out, g["keff"]):
+        assert abs(k - kr) <= 1e-5, (k, kr)
+
+
+def test_chunked_adjoint_matches_single_chunk():
+    prob = problem()
+    d = prob.dims
+    nfsr, G = d["nsr"] * d["fai"], d["G"]
+    rng = np.random.default_rng(3)
+    w = np.zeros((nfsr, G), np.float32)
+    w[rng.choice(nfsr, nfsr // 20, replace=False)] = 1.0
+    with sm.Solver(prob, sm.Options(tally="f64")) as s:
+        S = s.transport_sweep()
+        J1, gs1, gc1 = s.adjoint(w)
+    with sm.Solver(prob, sm.Options(tally="f64", max_records=S // 5 + 1)) as s:
+        s.transport_sweep()
+        J2, gs2, gc2 = s.adjoint(w)
+    assert J2 == pytest.approx(J1, rel=1e-9)
+    scale = max(np.abs(gs1).max(), np.abs(gc1).max())
+    assert np.abs(gs2 - gs1).max() <= 1e-9 * scale
+    assert np.abs(gc2 - gc1).max() <= 1e-9 * scale
