@@ -50,6 +50,14 @@ ADJOINT_WORKLOAD = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=
                         n_egroups=32, decompose=1, decomp_assemblies_ax=4, segments_per_track=20,
                         assembly_width=21.42, height=100.0, precision=0.01,
                         n_2D_source_regions_per_assembly=5000)
+# large single-GPU shape: 17x17 assemblies, 400 cm in 2 axial domains,
+# 40 coarse x 5 fine intervals, 128 (64 after halving) azimuthal x 12 polar,
+# 128 groups: 372 M tracks, forward psi 190.6 GB
+LARGE_WORKLOAD = dict(x_assemblies=17, y_assemblies=17, cai=40, fai=5, axial_exp=2,
+                      radial_ray_sep=0.05, axial_z_sep=0.25, n_azimuthal=128, n_polar_angles=12,
+                      n_egroups=128, decompose=1, decomp_assemblies_ax=2, segments_per_track=40,
+                      assembly_width=21.42, height=400.0, precision=0.01,
+                      n_2D_source_regions_per_assembly=5000)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "attenuate_traffic_plain_sweep_full_{tally}.json")
 METRIC = "3D segment\u00d7group integrations/s, plain vs batched; adjoint/forward time"
 
@@ -201,6 +209,55 @@ def batched_leg(sm, tally, iters=20):
             "tuning": {str(k): v for k, v in sorted(groups.items())}}
 
 
+def large_leg(sm, tally, adjoint=True):
+    """One power iteration at the large shape (the sweep's piece records
+    do not fit beside psi and run in chunks of whole z-stacks), then one
+    adjoint of the mean flux over a seeded 5 % of FSRs."""
+    t0 = time.perf_counter()
+    inp = sm.Input(**LARGE_WORKLOAD)
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=5)
+    G, T = inp.n_egroups, inp.ntracks
+    log(f"large: {inp.ntracks_2D} 2D tracks, {T} 3D tracks, {inp.n_source_regions_per_node} "
+        f"regions, G={G}, psi {T * G * 4 / 1e9:.1f} GB ({time.perf_counter() - t0:.1f} s)")
+    out = {"workload": "17x17 assemblies, 400 cm / 2 axial domains, 40 coarse x 5 fine axial "
+                       "intervals, 128 azimuthal x 12 polar, 128 groups, Poisson(40) segments",
+           "ntracks_3D": T, "psi_bytes": T * G * 4, "tally": tally}
+    with sm.Solver(prob, sm.Options(tally=tally)) as s:
+        log(f"large: device context ready ({time.perf_counter() - t0:.1f} s)")
+        t1 = time.perf_counter()
+        segs = s.transport_sweep()
+        t = s.timing()
+        s.renormalize_flux()
+        s.update_sources(1.0)
+        keff = s.compute_keff()
+        wall = time.perf_counter() - t1
+        log(f"large: sweep {t['sweep_ms']:.0f} ms in {t['chunks']} record chunks, {segs} pieces "
+            f"(attenuate {t['attenuate_ms']:.0f} ms, trace {t['trace_ms']:.0f} ms); iteration "
+            f"{wall:.1f} s, keff {keff:.6f}")
+        achieved = (24.0 * segs * G + 8.0 * T * G) / (t["attenuate_ms"] * 1e-3) / 1e9
+        out.update({"pieces": segs, "record_chunks": t["chunks"], "sweep_ms": t["sweep_ms"],
+                    "attenuate_ms": t["attenuate_ms"], "trace_ms": t["trace_ms"],
+                    "iteration_s": wall, "integrations_per_s": segs * G / wall,
+                    "sweep_integrations_per_s": segs * G / (t["sweep_ms"] * 1e-3),
+                    "attenuate_roofline_frac": achieved / HBM_PEAK_GBS, "keff": keff})
+        if adjoint:
+            nfsr = inp.n_source_regions_per_node * inp.fai
+            rng = np.random.default_rng(5)
+            w = np.zeros((nfsr, G), np.float32)
+            w[rng.choice(nfsr, nfsr // 20, replace=False)] = 1.0 / (nfsr // 20 * G)
+            s.transport_sweep()
+            fwd = s.timing()["sweep_ms"]
+            J, gs, gc = s.adjoint(w)
+            ta = s.timing()
+            log(f"large: adjoint {ta['adjoint_ms']:.0f} ms vs forward sweep {fwd:.0f} ms, J {J:.6e}")
+            out.update({"adjoint_ms": ta["adjoint_ms"], "forward_ms": fwd,
+                        "adjoint_over_forward": ta["adjoint_ms"] / fwd,
+                        "adjoint_finite": bool(np.isfinite(J) and np.isfinite(gs).all()
+                                               and np.isfinite(gc).all())})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +273,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched/plain leg")
+    ap.add_argument("--large", action="store_true",
+                    help="only the large shape (~200 GB of HBM): one iteration and one adjoint")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only time a plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -232,6 +291,12 @@ def main():
         sys.exit(2)
 
     import simplemoc_amd as sm
+
+    if args.large:
+        print(json.dumps({"metric": METRIC, "variant": "large", "unit": "integrations/s",
+                          "n_gpus": 1, "dtype": "fp32", "data": "synthetic",
+                          "large": large_leg(sm, args.tally)}), flush=True)
+        return
 
     t_setup = time.perf_counter()
     inp = sm.Input(**WORKLOAD)

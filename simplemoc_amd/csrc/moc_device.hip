@@ -264,9 +264,12 @@ __device__ __forceinline__ void add_nonzero(double *p, float v)
         atomicAdd(p, (double)v);
 }
 
-template <int LPT, bool QUAD, int EXPM>
+// lane l of a track owns groups l + r*LPT, r < GPL; PB pieces' loads
+// issue together
+template <int LPT, int GPL, bool QUAD, int EXPM>
 __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
 {
+    constexpr int PB = ADJ_PB / GPL > 0 ? ADJ_PB / GPL : 1;
     extern __shared__ float2 s_tab[];
     for (int e = threadIdx.x; e < a.N; e += blockDim.x)
         s_tab[e] = make_float2(a.table[2 * e], a.table[2 * e + 1]);
@@ -279,12 +282,17 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
     const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
                                  : (int)(threadIdx.x / LPT);
     const int G = a.G;
-    const int g = min(l, G - 1);
+    int gg[GPL];
+    bool live[GPL];
+#pragma unroll
+    for (int r = 0; r < GPL; r++) {
+        gg[r] = min(l + r * LPT, G - 1);   // clamped group for unconditional loads
+        live[r] = l + r * LPT < G;
+    }
     const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
     const int N = a.N;
-    const bool live = l < G;
     const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
-    float *tape = a.tape + ((long long)blockIdx.x * TPB + slot) * a.tape_stride + g;
+    float *tape = a.tape + ((long long)blockIdx.x * TPB + slot) * a.tape_stride;
     double Jacc = 0.0;
     const long long nblk = (a.t_end - a.t_begin + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
@@ -298,59 +306,76 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
         const long long q0 = has ? a.off[tt] - a.rec_base : 0;
         const int P = has ? (int)(a.off[tt + 1] - a.off[tt]) : 0;
-        float ps = a.psi[tt * G + g] * pscale;
+        float ps[GPL];
+#pragma unroll
+        for (int r = 0; r < GPL; r++)
+            ps[r] = a.psi[tt * G + gg[r]] * pscale;
         // forward: psi entering each piece onto the tape
-        for (int p0 = 0; p0 < P; p0 += ADJ_PB) {
-            float4 r[ADJ_PB], c[ADJ_PB];
+        for (int p0 = 0; p0 < P; p0 += PB) {
+            float4 rc[PB], c[PB][GPL];
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++)
-                r[k] = a.rec[q0 + min(p0 + k, P - 1)];
+            for (int k = 0; k < PB; k++)
+                rc[k] = a.rec[q0 + min(p0 + k, P - 1)];
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++)
-                c[k] = a.coef[(size_t)__float_as_uint(r[k].z) * G + g];
+            for (int k = 0; k < PB; k++)
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++) {
+                for (int r = 0; r < GPL; r++)
+                    c[k][r] = a.coef[(size_t)__float_as_uint(rc[k].z) * G + gg[r]];
+#pragma unroll
+            for (int k = 0; k < PB; k++) {
                 if (p0 + k < P) {
-                    if (live)
-                        tape[(long long)(p0 + k) * G] = ps;
-                    piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k], r[k].x, r[k].y, mu,
-                                              mu2, ps);
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        if (live[r])
+                            tape[(long long)(p0 + k) * G + gg[r]] = ps[r];
+                        piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k][r], rc[k].x, rc[k].y,
+                                                  mu, mu2, ps[r]);
+                    }
                 }
             }
         }
         // reverse
-        float lam = 0.f;
-        for (int p1 = P - 1; p1 >= 0; p1 -= ADJ_PB) {
-            float4 r[ADJ_PB], c[ADJ_PB];
-            float wv_[ADJ_PB], psp[ADJ_PB];
+        float lam[GPL];
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++)
-                r[k] = a.rec[q0 + max(p1 - k, 0)];
+        for (int r = 0; r < GPL; r++)
+            lam[r] = 0.f;
+        for (int p1 = P - 1; p1 >= 0; p1 -= PB) {
+            float4 rc[PB], c[PB][GPL];
+            float wv_[PB][GPL], psp[PB][GPL];
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++) {
-                const size_t e = (size_t)__float_as_uint(r[k].z) * G + g;
-                c[k] = a.coef[e];
-                wv_[k] = a.w[e];
-                psp[k] = tape[(long long)max(p1 - k, 0) * G];
-            }
+            for (int k = 0; k < PB; k++)
+                rc[k] = a.rec[q0 + max(p1 - k, 0)];
 #pragma unroll
-            for (int k = 0; k < ADJ_PB; k++) {
+            for (int k = 0; k < PB; k++)
+#pragma unroll
+                for (int r = 0; r < GPL; r++) {
+                    const size_t e = (size_t)__float_as_uint(rc[k].z) * G + gg[r];
+                    c[k][r] = a.coef[e];
+                    wv_[k][r] = a.w[e];
+                    psp[k][r] = tape[(long long)max(p1 - k, 0) * G + gg[r]];
+                }
+#pragma unroll
+            for (int k = 0; k < PB; k++) {
                 if (p1 - k >= 0) {
-                    const float bw = wv_[k] * wt;
-                    const PieceGrad o = piece_reverse<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k],
-                                                                  r[k].x, r[k].y, mu, mu2, psp[k],
-                                                                  bw, lam);
-                    if (live) {
-                        Jacc += (double)bw * (double)o.I;
-                        double *ap = a.acc + (size_t)__float_as_uint(r[k].z) * 4 * G + g;
-                        add_nonzero(ap, o.gc0);
-                        if (QUAD) {
-                            add_nonzero(ap + G, o.gc1);
-                            add_nonzero(ap + 2 * G, o.gc2);
+                    const uint32_t fsr = __float_as_uint(rc[k].z);
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        const float bw = wv_[k][r] * wt;
+                        const PieceGrad o = piece_reverse<QUAD, EXPM>(
+                            s_tab, dx, hdx, maxv, N, c[k][r], rc[k].x, rc[k].y, mu, mu2, psp[k][r],
+                            bw, lam[r]);
+                        if (live[r]) {
+                            Jacc += (double)bw * (double)o.I;
+                            double *ap = a.acc + (size_t)fsr * 4 * G + gg[r];
+                            add_nonzero(ap, o.gc0);
+                            if (QUAD) {
+                                add_nonzero(ap + G, o.gc1);
+                                add_nonzero(ap + 2 * G, o.gc2);
+                            }
+                            add_nonzero(ap + 3 * G, o.gs);
                         }
-                        add_nonzero(ap + 3 * G, o.gs);
+                        lam[r] = o.lam;
                     }
-                    lam = o.lam;
                 }
             }
         }
@@ -1216,7 +1241,7 @@ static int materialize_psi(moc_ctx *c)
 
 static int ensure_records(moc_ctx *c, long long S, long long budget)
 {
-    if (S <= c->rec_cap)
+    if (S <= c->rec_cap && c->rec_cap <= budget)
         return MOC_OK;
     long long cap = S + S / 8 + 1024;
     if (cap > budget)
@@ -1252,8 +1277,9 @@ struct Chunk {
 };
 
 // Records one chunk may hold: the caller's max_records, else what device
-// memory allows next to everything else (1 GiB kept free), else all of S.
-static int records_budget(moc_ctx *c, long long S, long long *budget)
+// memory allows next to everything else (reserve bytes kept free), else
+// all of S.
+static int records_budget(moc_ctx *c, long long S, long long reserve, long long *budget)
 {
     if (c->opt.max_records > 0) {
         *budget = c->opt.max_records;
@@ -1261,7 +1287,7 @@ static int records_budget(moc_ctx *c, long long S, long long *budget)
     }
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-    long long avail = ((long long)free_b + c->rec_cap * (long long)sizeof(float4) - (1LL << 30)) /
+    long long avail = ((long long)free_b + c->rec_cap * (long long)sizeof(float4) - reserve) /
                       (long long)sizeof(float4);
     const long long want = S + S / 8 + 1024;
     *budget = avail >= want ? want : avail;
@@ -1273,7 +1299,8 @@ static int records_budget(moc_ctx *c, long long S, long long *budget)
 // Count pass and offsets of the next sweep (z unchanged), then the chunks:
 // one when all records fit, else runs of whole stacks of at most the
 // budget's records each.
-static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out)
+static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
+                      long long reserve = 1LL << 30)
 {
     const long long nst = c->nt2 * c->np;
     TraceArgs ta = trace_args(c);
@@ -1286,7 +1313,7 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out)
     HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     long long budget = 0;
-    TRY(records_budget(c, S, &budget));
+    TRY(records_budget(c, S, reserve, &budget));
     chunks.clear();
     if (S <= budget) {
         chunks.push_back({0, nst, 0, S});
@@ -1311,8 +1338,8 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out)
         if (biggest > budget && c->opt.max_records == 0)
             return moc_set_error(MOC_ERR_NOMEM, "one z-stack needs %lld piece records, more than "
                                  "device memory holds", biggest);
-        // exact fit: chunks are reused, never grown
-        if (biggest > c->rec_cap) {
+        // exact fit: every chunk reuses the one buffer
+        if (biggest > c->rec_cap || c->rec_cap > budget) {
             if (c->d_rec)
                 HIP_TRY(hipFree(c->d_rec));
             c->d_rec = nullptr;
@@ -1826,15 +1853,24 @@ extern "C" int moc_get_coefficients(moc_ctx *c, float *coef3, float *sigma)
     return MOC_OK;
 }
 
-template <int LPT, bool QUAD>
+template <int LPT, int GPL, bool QUAD>
 static void launch_adj_lpt(moc_ctx *c, const AdjArgs &a, int grid, size_t lds)
 {
     if (c->opt.exp_mode == MOC_EXP_TABLE)
-        hipLaunchKernelGGL((k_adjoint<LPT, QUAD, MOC_EXP_TABLE>), dim3(grid), dim3(64 * ADJ_WAVES), lds,
-                           c->stream, a);
-    else
-        hipLaunchKernelGGL((k_adjoint<LPT, QUAD, MOC_EXP_INTRINSIC>), dim3(grid), dim3(64 * ADJ_WAVES),
+        hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_TABLE>), dim3(grid), dim3(64 * ADJ_WAVES),
                            lds, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_INTRINSIC>), dim3(grid),
+                           dim3(64 * ADJ_WAVES), lds, c->stream, a);
+}
+
+template <int LPT, int GPL>
+static void launch_adj(moc_ctx *c, const AdjArgs &a, int grid, size_t lds)
+{
+    if (c->quad)
+        launch_adj_lpt<LPT, GPL, true>(c, a, grid, lds);
+    else
+        launch_adj_lpt<LPT, GPL, false>(c, a, grid, lds);
 }
 
 extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *grad_sigma,
@@ -1842,15 +1878,12 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
 {
     if (!c || !weights)
         return moc_set_error(MOC_ERR_INVALID, "moc_adjoint: null argument");
-    if (c->gpl != 1)
-        return moc_set_error(MOC_ERR_INVALID, "moc_adjoint supports up to 64 energy groups (got %d)",
+    if (c->gpl > 4)
+        return moc_set_error(MOC_ERR_INVALID, "moc_adjoint supports up to 256 energy groups (got %d)",
                              c->G);
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     TRY(prep_coefficients(c, nullptr));
-    long long S = 0;
-    std::vector<Chunk> chunks;
-    TRY(plan_trace(c, chunks, &S));
     if (!c->d_adj_w) {
         TRY(dalloc(&c->d_adj_w, (size_t)c->nval));
         TRY(dalloc(&c->d_adj_gs, (size_t)c->nsr * c->G));
@@ -1864,6 +1897,10 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         TRY(dalloc(&c->d_adj_rtmp, rb / 8 + 1));
         c->adj_rtmp_bytes = rb;
     }
+    // the records leave room for the largest tape
+    long long S = 0;
+    std::vector<Chunk> chunks;
+    TRY(plan_trace(c, chunks, &S, (1LL << 30) + ADJ_TAPE_FLOATS * (long long)sizeof(float)));
     // the longest track sizes the tape
     size_t rb = c->adj_rtmp_bytes;
     HIP_TRY(rocprim::reduce(c->d_adj_rtmp, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
@@ -1926,20 +1963,17 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         aa.t_begin = ch.s0 * c->zs;
         aa.t_end = ch.s1 * c->zs;
         HIP_TRY(hipEventRecord(c->ev[5], c->stream));
-#define MOC_ADJ(L)                                   \
-    do {                                             \
-        if (c->quad)                                 \
-            launch_adj_lpt<L, true>(c, aa, (int)grid, lds); \
-        else                                         \
-            launch_adj_lpt<L, false>(c, aa, (int)grid, lds); \
-    } while (0)
         switch (c->lpt) {
-        case 8: MOC_ADJ(8); break;
-        case 16: MOC_ADJ(16); break;
-        case 32: MOC_ADJ(32); break;
-        default: MOC_ADJ(64); break;
+        case 8: launch_adj<8, 1>(c, aa, (int)grid, lds); break;
+        case 16: launch_adj<16, 1>(c, aa, (int)grid, lds); break;
+        case 32: launch_adj<32, 1>(c, aa, (int)grid, lds); break;
+        default:
+            switch (c->gpl) {
+            case 1: launch_adj<64, 1>(c, aa, (int)grid, lds); break;
+            case 2: launch_adj<64, 2>(c, aa, (int)grid, lds); break;
+            default: launch_adj<64, 4>(c, aa, (int)grid, lds); break;
+            }
         }
-#undef MOC_ADJ
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[6], c->stream));
         HIP_TRY(hipEventSynchronize(c->ev[6]));

@@ -30,8 +30,8 @@ CONFIG4_SMALL = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=2, 
                      height=100.0, precision=0.01, n_2D_source_regions_per_assembly=400)
 
 
-def build(exp="intrinsic", axial_exp=2, seed=4):
-    inp = make_input(**dict(CONFIG4_SMALL, axial_exp=axial_exp))
+def build(exp="intrinsic", axial_exp=2, seed=4, **over):
+    inp = make_input(**dict(CONFIG4_SMALL, axial_exp=axial_exp, **over))
     prob = sm.Problem.generate(inp, seed=seed, alloc_psi=True)
     d = prob.dims
     rng = np.random.default_rng(seed)
@@ -58,6 +58,27 @@ def test_adjoint_matches_fp64_adjoint(exp, axial_exp):
     scale = max(np.abs(gs64).max(), np.abs(gc64).max())
     print(f"{exp}/{axial_exp}: J {J:.6e}, |grad| max {scale:.3e}, "
           f"max err sigma {np.abs(gs - gs64).max() / scale:.2e}, coef {np.abs(gc - gc64).max() / scale:.2e}")
+    assert np.abs(gs - gs64).max() <= 1e-5 * scale
+    assert np.abs(gc - gc64).max() <= 1e-5 * scale
+
+
+@pytest.mark.parametrize("G", [96, 128])
+def test_adjoint_several_groups_per_lane(G):
+    """More groups than lanes: each lane carries two groups (96: the
+    second group of half the lanes does not exist)."""
+    prob, solver, w, _ = build("table", 2, seed=9, n_egroups=G, radial_ray_sep=8.0,
+                               n_2D_source_regions_per_assembly=100)
+    with solver:
+        assert solver.launch() == (64, 2)
+        J, gs, gc = solver.adjoint(w)
+        coef, sigma = solver.coefficients()
+    o = oracle_for(prob)
+    J64, gs64, gc64 = o.adjoint_f64(coef.astype(np.float64), sigma.astype(np.float64),
+                                    w.astype(np.float64), exp_mode=po.EXP_TABLE)
+    assert J == pytest.approx(J64, rel=1e-5)
+    scale = max(np.abs(gs64).max(), np.abs(gc64).max())
+    print(f"G={G}: max err sigma {np.abs(gs - gs64).max() / scale:.2e}, "
+          f"coef {np.abs(gc - gc64).max() / scale:.2e}")
     assert np.abs(gs - gs64).max() <= 1e-5 * scale
     assert np.abs(gc - gc64).max() <= 1e-5 * scale
 
