@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched/plain leg")
     ap.add_argument("--large", action="store_true",
                     help="only the large shape (~200 GB of HBM): one iteration and one adjoint")
+    ap.add_argument("--only", choices=["adjoint", "batched"],
+                    help="run only this leg and print its figures (for profiling)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only time a plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -292,6 +294,11 @@ def main():
 
     import simplemoc_amd as sm
 
+    if args.only:
+        leg = adjoint_leg(sm) if args.only == "adjoint" else batched_leg(sm, args.tally)
+        print(json.dumps({"metric": METRIC, "variant": args.only, "n_gpus": 1,
+                          args.only: leg}), flush=True)
+        return
     if args.large:
         print(json.dumps({"metric": METRIC, "variant": "large", "unit": "integrations/s",
                           "n_gpus": 1, "dtype": "fp32", "data": "synthetic",

@@ -161,6 +161,7 @@ struct AdjArgs {
     float maxv;
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch
+    const uint32_t *wmask;      // bit fsr: some group of FSR fsr has a nonzero weight
 };
 
 template <int EXPM>
@@ -305,7 +306,24 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         const float mu2 = mu * mu;
         const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
         const long long q0 = has ? a.off[tt] - a.rec_base : 0;
-        const int P = has ? (int)(a.off[tt + 1] - a.off[tt]) : 0;
+        const int P_all = has ? (int)(a.off[tt + 1] - a.off[tt]) : 0;
+        // Pieces after the track's last weighted one carry no weight and a
+        // zero adjoint, so they add nothing to J or to any gradient: the
+        // track is processed up to that piece only.  The track's lanes scan
+        // its records LPT at a time against the FSR weight mask.
+        int last = -1;
+        for (int p0 = 0; p0 < P_all; p0 += LPT) {
+            const int p = p0 + l;
+            if (p < P_all) {
+                const uint32_t f = __float_as_uint(a.rec[q0 + p].z);
+                if ((a.wmask[f >> 5] >> (f & 31)) & 1u)
+                    last = p;
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < LPT; o <<= 1)
+            last = max(last, __shfl_xor(last, o, 64));
+        const int P = last + 1;
         float ps[GPL];
 #pragma unroll
         for (int r = 0; r < GPL; r++)
@@ -392,6 +410,26 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         for (int q = 0; q < ADJ_WAVES; q++)
             s += s_j[q];
         atomicAdd(a.J, s);
+    }
+}
+
+// bit fsr of mask: some group of FSR fsr has a nonzero weight
+__global__ void k_adj_mask(const float *__restrict__ w, uint32_t *__restrict__ mask, long long nfsr,
+                           int G)
+{
+    for (long long m = blockIdx.x * (long long)blockDim.x + threadIdx.x; m < (nfsr + 31) / 32;
+         m += (long long)gridDim.x * blockDim.x) {
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; b++) {
+            const long long f = m * 32 + b;
+            if (f >= nfsr)
+                break;
+            bool any = false;
+            for (int g = 0; g < G && !any; g++)
+                any = w[f * G + g] != 0.0f;
+            bits |= (uint32_t)any << b;
+        }
+        mask[m] = bits;
     }
 }
 
@@ -768,6 +806,7 @@ struct moc_ctx {
     float *d_adj_w = nullptr;
     double *d_adj_gs = nullptr, *d_adj_gc = nullptr, *d_adj_J = nullptr, *d_adj_acc = nullptr;
     int *d_adj_maxp = nullptr;
+    uint32_t *d_adj_mask = nullptr;
     double *d_adj_rtmp = nullptr;
     size_t adj_rtmp_bytes = 0;
     float *d_adj_tape = nullptr;
@@ -825,7 +864,7 @@ static void free_all(moc_ctx *c)
                     c->d_stream, c->d_scan_tmp, c->d_partial,
                     c->d_gres, c->d_rres, c->d_scal, c->d_err, c->d_user_c, c->d_user_s,
                     c->d_adj_w, c->d_adj_gs, c->d_adj_gc, c->d_adj_J, c->d_adj_acc,
-                    c->d_adj_maxp, c->d_adj_rtmp, c->d_adj_tape};
+                    c->d_adj_maxp, c->d_adj_rtmp, c->d_adj_tape, c->d_adj_mask};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1891,6 +1930,7 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         TRY(dalloc(&c->d_adj_acc, (size_t)c->nval * 4));
         TRY(dalloc(&c->d_adj_J, 1));
         TRY(dalloc(&c->d_adj_maxp, 1));
+        TRY(dalloc(&c->d_adj_mask, (size_t)(c->nfsr + 31) / 32));
         size_t rb = 0;
         HIP_TRY(rocprim::reduce(nullptr, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
                                 rocprim::maximum<int>(), c->stream));
@@ -1927,6 +1967,8 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         c->adj_tape_cap = grid * per_block;
     }
     HIP_TRY(hipMemcpyAsync(c->d_adj_w, weights, (size_t)c->nval * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_adj_mask, dim3(launch_grid((c->nfsr + 31) / 32, 256)), dim3(256), 0,
+                       c->stream, c->d_adj_w, c->d_adj_mask, c->nfsr, c->G);
     hipLaunchKernelGGL(k_fill_d, dim3(launch_grid(c->nval * 4, 256)), dim3(256), 0, c->stream,
                        c->d_adj_acc, c->nval * 4);
     hipLaunchKernelGGL(k_fill_d, dim3(1), dim3(64), 0, c->stream, c->d_adj_J, 1LL);
@@ -1941,6 +1983,7 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     aa.acc = c->d_adj_acc;
     aa.tape = c->d_adj_tape;
     aa.tape_stride = (long long)maxp * c->G;
+    aa.wmask = c->d_adj_mask;
     aa.J = c->d_adj_J;
     aa.pw = c->d_pw;
     aa.az = c->d_az;
