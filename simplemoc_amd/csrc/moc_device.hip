@@ -364,14 +364,19 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
             for (int k = 0; k < PB; k++)
                 rc[k] = a.rec[q0 + max(p1 - k, 0)];
 #pragma unroll
-            for (int k = 0; k < PB; k++)
+            for (int k = 0; k < PB; k++) {
+                // weights are gathered only for FSRs that have some (the
+                // same for all lanes of the track)
+                const uint32_t f = __float_as_uint(rc[k].z);
+                const bool weighted = (a.wmask[f >> 5] >> (f & 31)) & 1u;
 #pragma unroll
                 for (int r = 0; r < GPL; r++) {
-                    const size_t e = (size_t)__float_as_uint(rc[k].z) * G + gg[r];
+                    const size_t e = (size_t)f * G + gg[r];
                     c[k][r] = a.coef[e];
-                    wv_[k][r] = a.w[e];
+                    wv_[k][r] = weighted ? a.w[e] : 0.0f;
                     psp[k][r] = tape[(long long)max(p1 - k, 0) * G + gg[r]];
                 }
+            }
 #pragma unroll
             for (int k = 0; k < PB; k++) {
                 if (p1 - k >= 0) {
@@ -1316,8 +1321,9 @@ struct Chunk {
 };
 
 // Records one chunk may hold: the caller's max_records, else what device
-// memory allows next to everything else (reserve bytes kept free), else
-// all of S.
+// memory allows next to everything else (reserve bytes kept free: the
+// runtime also allocates the scratch of kernels with a private segment,
+// up to ~2.2 GB for the pairwise-sum reductions), else all of S.
 static int records_budget(moc_ctx *c, long long S, long long reserve, long long *budget)
 {
     if (c->opt.max_records > 0) {
@@ -1338,8 +1344,10 @@ static int records_budget(moc_ctx *c, long long S, long long reserve, long long 
 // Count pass and offsets of the next sweep (z unchanged), then the chunks:
 // one when all records fit, else runs of whole stacks of at most the
 // budget's records each.
+constexpr long long RECORD_RESERVE = 4LL << 30;
+
 static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
-                      long long reserve = 1LL << 30)
+                      long long reserve = RECORD_RESERVE)
 {
     const long long nst = c->nt2 * c->np;
     TraceArgs ta = trace_args(c);
@@ -1436,6 +1444,19 @@ static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit)
     return MOC_OK;
 }
 
+// A chunked sweep's records take what device memory had left; they are
+// given back afterwards so that later kernels (and their scratch) fit.
+static int release_records(moc_ctx *c)
+{
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->d_rec)
+        HIP_TRY(hipFree(c->d_rec));
+    c->d_rec = nullptr;
+    c->rec_cap = 0;
+    c->rec_t_begin = c->rec_t_end = 0;
+    return MOC_OK;
+}
+
 // what the record buffer holds after write_chunk
 static void note_chunk(moc_ctx *c, const Chunk &ch)
 {
@@ -1513,6 +1534,8 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     }
     c->psi_pending = false;
     c->chunks = (long)chunks.size();
+    if (chunks.size() > 1)
+        TRY(release_records(c));
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     // (6) flux = (float)accumulator
     if (c->opt.tally == MOC_TALLY_F64) {
@@ -1940,7 +1963,7 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     // the records leave room for the largest tape
     long long S = 0;
     std::vector<Chunk> chunks;
-    TRY(plan_trace(c, chunks, &S, (1LL << 30) + ADJ_TAPE_FLOATS * (long long)sizeof(float)));
+    TRY(plan_trace(c, chunks, &S, RECORD_RESERVE + ADJ_TAPE_FLOATS * (long long)sizeof(float)));
     // the longest track sizes the tape
     size_t rb = c->adj_rtmp_bytes;
     HIP_TRY(rocprim::reduce(c->d_adj_rtmp, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
@@ -2024,6 +2047,8 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         HIP_TRY(hipEventElapsedTime(&t, c->ev[5], c->ev[6]));
         ms_k += t;
     }
+    if (chunks.size() > 1)
+        TRY(release_records(c));
     hipLaunchKernelGGL(k_adj_finish, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
                        c->d_adj_acc, c->d_adj_gc, c->d_adj_gs, (long long)c->nsr, c->G, c->fai);
     HIP_TRY(hipGetLastError());
