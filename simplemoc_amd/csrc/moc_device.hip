@@ -47,28 +47,32 @@ using namespace moc;
 
 // ---------------------------------------------------------------------
 // exact reference pairwise sum (src/utils.c:29-45) over an element functor:
-// halves at n/2 until n <= 16, then sums sequentially.  One function per
-// depth level keeps the call graph acyclic.
+// halves at n/2 until n <= 16, then sums sequentially.  The tree is
+// unrolled at compile time to depth D, which covers n <= 16 * 2^D (the
+// node sizes at depth d are floor or ceil of n / 2^d); no call stack, so
+// the kernels need no scratch memory.
 template <int D, class F>
-__device__ __noinline__ float pw_sum(const F &f, long long base, long long n)
+__device__ __forceinline__ float pairwise(const F &f, long long base, long long n)
 {
-    if (n <= 16 || D == 0) {
+    if constexpr (D == 0) {
         float s = 0.0f;
         for (long long i = 0; i < n; ++i)
             s += f(base + i);
         return s;
+    } else {
+        if (n <= 16) {
+            float s = 0.0f;
+            for (long long i = 0; i < n; ++i)
+                s += f(base + i);
+            return s;
+        }
+        const long long h = n / 2;
+        return pairwise<D - 1>(f, base, h) + pairwise<D - 1>(f, base + h, n - h);
     }
-    long long h = n / 2;
-    if constexpr (D > 0)
-        return pw_sum<D - 1>(f, base, h) + pw_sum<D - 1>(f, base + h, n - h);
-    return 0.0f;
 }
-// depth 27 covers n up to 16 * 2^27
-template <class F>
-__device__ inline float pairwise(const F &f, long long base, long long n)
-{
-    return pw_sum<27>(f, base, n);
-}
+constexpr int PW_GROUP_DEPTH = 5;    // sums over groups: G <= 512
+constexpr int PW_FINE_DEPTH = 6;     // sums over fine intervals: fai <= 1024
+constexpr int MAX_FAI = 16 << PW_FINE_DEPTH;
 
 // ---------------------------------------------------------------------
 // (1)-(3) the sweep kernels of one problem (device code in moc_sweep.h)
@@ -595,9 +599,9 @@ __global__ void k_update_sources(const float *__restrict__ flux, float *__restri
         const float *phi = flux + fsr * G;
         const float *X = xs + m * G * 3;
         const float *S = scT + m * (long long)G * G + g;
-        float fission = pairwise([&](long long q) { return phi[q] * X[q * 3]; }, 0, G);
+        float fission = pairwise<PW_GROUP_DEPTH>([&](long long q) { return phi[q] * X[q * 3]; }, 0, G);
         fission *= inverse_k;
-        float scatter = pairwise([&](long long q) { return S[q * G] * phi[q]; }, 0, G);
+        float scatter = pairwise<PW_GROUP_DEPTH>([&](long long q) { return S[q * G] * phi[q]; }, 0, G);
         float chi = X[g * 3 + 2];
         float newSrc = (float)((double)(fission * chi + scatter) / (4.0 * M_PI));
         float oldSrc = src[e];
@@ -607,18 +611,23 @@ __global__ void k_update_sources(const float *__restrict__ flux, float *__restri
 }
 
 // residuals[i] = pairwise over fai of pairwise over G (src/solver.c:1301-1303)
-__global__ void k_residual_regions(const float *__restrict__ gres, float *__restrict__ rres,
-                                   long long nsr, int G, int fai)
+// one wavefront per region: lanes sum the rows (fine intervals) over the
+// groups into LDS, lane 0 sums the rows
+__global__ __launch_bounds__(64) void k_residual_regions(const float *__restrict__ gres,
+                                                         float *__restrict__ rres, long long nsr,
+                                                         int G, int fai)
 {
-    for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nsr;
-         r += (long long)gridDim.x * blockDim.x) {
+    __shared__ float s_row[MAX_FAI];
+    for (long long r = blockIdx.x; r < nsr; r += gridDim.x) {
         const float *base = gres + r * fai * G;
-        rres[r] = pairwise(
-            [&](long long j) {
-                const float *row = base + j * G;
-                return pairwise([&](long long g) { return row[g]; }, 0, G);
-            },
-            0, fai);
+        for (int j = threadIdx.x; j < fai; j += blockDim.x) {
+            const float *row = base + (long long)j * G;
+            s_row[j] = pairwise<PW_GROUP_DEPTH>([&](long long g) { return row[g]; }, 0, G);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            rres[r] = pairwise<PW_FINE_DEPTH>([&](long long j) { return s_row[j]; }, 0, fai);
+        __syncthreads();
     }
 }
 
@@ -1016,6 +1025,9 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
     if (c->nsr > 16LL * PW_MAX_NODES * 2)
         return fail(moc_set_error(MOC_ERR_INVALID, "%lld source regions exceed the %lld supported",
                                   c->nsr, 16LL * PW_MAX_NODES * 2));
+    if (c->fai > MAX_FAI || c->G > 16 << PW_GROUP_DEPTH)
+        return fail(moc_set_error(MOC_ERR_INVALID, "%d fine intervals and %d groups exceed the "
+                                  "supported %d and %d", c->fai, c->G, MAX_FAI, 16 << PW_GROUP_DEPTH));
     if ((size_t)c->table_N * sizeof(float2) > 65536)
         return fail(moc_set_error(MOC_ERR_INVALID, "exponential table of %d cells exceeds 64 KiB of LDS",
                                   c->table_N));
@@ -1662,7 +1674,7 @@ extern "C" int moc_update_sources(moc_ctx *c, float keff, float *residual)
     hipLaunchKernelGGL(k_update_sources, dim3(launch_grid(c->nval, 128)), dim3(128), 0, c->stream,
                        c->d_flux, c->d_src, c->d_xs, c->d_scT, c->d_mat, c->d_gres, c->nval, c->G,
                        c->fai, inverse_k);
-    hipLaunchKernelGGL(k_residual_regions, dim3(launch_grid(c->nsr, 64)), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(k_residual_regions, dim3(launch_grid(c->nsr, 1, 65536)), dim3(64), 0, c->stream,
                        c->d_gres, c->d_rres, c->nsr, c->G, c->fai);
     hipLaunchKernelGGL(k_pairwise_block, dim3(1), dim3(1024), 0, c->stream, c->d_rres, c->nsr,
                        c->d_scal + SC_RESID);
