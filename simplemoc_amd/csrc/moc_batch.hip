@@ -8,16 +8,17 @@
 // are computed once.  Per iteration, for all problems together:
 //
 //   k_bseed       acc = flux                      (FP64 tally only)
-//   k_btrace<0>   one workgroup per (problem, z-stack): piece counts
-//   k_scan_*      piece offsets over every track of every problem
-//   k_btrace<1>   piece records into one buffer, z heights committed
+//   k_btrace      one workgroup per (problem, z-stack): the axial trace
+//                 emits each track's records into the track's own region of
+//                 one buffer (regions sized once by the piece bound, so no
+//                 count pass or scan), piece counts, z heights committed
 //   k_batt        persistent attenuation: workgroups pull work items
 //                 (a run of one problem's tracks) from a queue ordered by
 //                 piece count, longest first; each item runs the launch
 //                 variant (lanes per track, groups per lane, pieces per
 //                 batch) chosen for its problem
 //   k_bfinal      flux = (float)acc               (FP64 tally only)
-//   k_bstep       sweep counters and piece totals
+//   k_bstep       sweep counters
 //
 // Nothing in an iteration needs the host: the record buffer is sized once
 // by a bound valid for any z state, and the sweep counters live on the
@@ -133,8 +134,10 @@ __global__ void k_bfinal(const double *__restrict__ acc, float *__restrict__ flu
         flux[e] = (float)acc[e];
 }
 
-// global stack s -> (problem, local stack); commit < 0 keeps the problem's own flag
-template <bool WRITE, int KPT>
+// global stack s -> (problem, local stack).  A sweep (commit = 1) commits
+// the z heights and adds its pieces to the problem's total; commit = 0
+// leaves both alone.
+template <int KPT>
 __global__ __launch_bounds__(256) void k_btrace(const TraceArgs *__restrict__ probs,
                                                  const int *__restrict__ stack_prob,
                                                  const long long *__restrict__ stack_first,
@@ -144,101 +147,21 @@ __global__ __launch_bounds__(256) void k_btrace(const TraceArgs *__restrict__ pr
     for (long long s = blockIdx.x; s < nst; s += gridDim.x) {
         const int p = stack_prob[s];
         TraceArgs a = probs[p];
-        if (commit >= 0)
-            a.commit = commit;
+        a.commit = commit;
+        if (!commit)
+            a.seg_total = nullptr;
         const uint64_t skey = sweep_key(a.seed, *a.sweep_ctr);
-        trace_stack<WRITE, KPT>(a, s - stack_first[p], skey, s_w);
+        trace_stack<TRACE_EMIT, KPT>(a, s - stack_first[p], skey, s_w);
     }
-}
-
-// exclusive scan of n ints into int64, in tiles of SCAN_TILE
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-
-__device__ __forceinline__ long long block_scan_ll(long long v, long long *s_w, long long *total)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    long long x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long y = __shfl_up(x, o, 64);
-        if (lane >= o)
-            x += y;
-    }
-    if (lane == 63)
-        s_w[w] = x;
-    __syncthreads();
-    long long pre = 0, tot = 0;
-    for (int q = 0; q < nw; q++) {
-        const long long t = s_w[q];
-        pre += (q < w) ? t : 0;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + x - v;
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const int *__restrict__ cnt,
-                                                             long long *__restrict__ off,
-                                                             long long n, long long *__restrict__ tile_sum)
-{
-    __shared__ long long s_w[SCAN_THREADS / 64];
-    const long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
-    long long v[SCAN_ITEMS];
-    long long run = 0;
-#pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) {
-        v[q] = (base + q < n) ? (long long)cnt[base + q] : 0;
-        run += v[q];
-    }
-    long long tot;
-    long long pre = block_scan_ll(run, s_w, &tot);
-#pragma unroll
-    for (int q = 0; q < SCAN_ITEMS; q++) {
-        if (base + q < n)
-            off[base + q] = pre;
-        pre += v[q];
-    }
-    if (threadIdx.x == 0)
-        tile_sum[blockIdx.x] = tot;
-}
-
-// one workgroup: exclusive scan of the tile sums in place
-__global__ __launch_bounds__(1024) void k_scan_sums(long long *__restrict__ tile_sum, long long nt)
-{
-    __shared__ long long s_w[16];
-    long long carry = 0;
-    for (long long b = 0; b < nt; b += blockDim.x) {
-        const long long e = b + threadIdx.x;
-        const long long v = e < nt ? tile_sum[e] : 0;
-        long long tot;
-        const long long pre = block_scan_ll(v, s_w, &tot);
-        if (e < nt)
-            tile_sum[e] = carry + pre;
-        carry += tot;
-    }
-}
-
-__global__ void k_scan_add(long long *__restrict__ off, long long n, const long long *__restrict__ tile_sum)
-{
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
-         e += (long long)gridDim.x * blockDim.x)
-        off[e] += tile_sum[e / SCAN_TILE];
 }
 
 __global__ void k_reset_head(unsigned *head) { if (threadIdx.x == 0 && blockIdx.x == 0) head[0] = 0u; }
 
-// sweep counters +1; per-problem piece totals += this sweep's pieces
-__global__ void k_bstep(long long *__restrict__ sweep, long long *__restrict__ segs,
-                        const long long *__restrict__ off, const long long *__restrict__ track_first,
-                        int nprob)
+// sweep counters +1
+__global__ void k_bstep(long long *__restrict__ sweep, int nprob)
 {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nprob; p += gridDim.x * blockDim.x)
         sweep[p] += 1;
-        segs[p] += off[track_first[p + 1]] - off[track_first[p]];
-    }
 }
 
 template <int LPT, int GPL, int PB, bool T64, int EXPM>
@@ -359,7 +282,7 @@ struct moc_batch {
     float4 *d_coef = nullptr;
     double *d_acc = nullptr;
     int *d_cnt = nullptr;
-    long long *d_off = nullptr, *d_tile = nullptr;
+    long long *d_off = nullptr;     // per-track record regions (prefix of the bounds)
     float4 *d_rec = nullptr;
     long long *d_sweep = nullptr, *d_segs = nullptr, *d_track_first = nullptr, *d_stack_first = nullptr;
     int *d_stack_prob = nullptr;
@@ -385,7 +308,7 @@ void free_batch(moc_batch *b)
         (void)hipGraphDestroy(b->graph);
     void *ptrs[] = {b->d_segoff, b->d_seglen, b->d_az, b->d_mu, b->d_cosd, b->d_sind, b->d_pw,
                     b->d_z, b->d_psi, b->d_src, b->d_sig, b->d_flux, b->d_table, b->d_coef,
-                    b->d_acc, b->d_cnt, b->d_off, b->d_tile, b->d_rec, b->d_sweep, b->d_segs,
+                    b->d_acc, b->d_cnt, b->d_off, b->d_rec, b->d_sweep, b->d_segs,
                     b->d_track_first, b->d_stack_first, b->d_stack_prob, b->d_tr, b->d_at,
                     b->d_items, b->d_tune_items, b->d_head, b->d_err};
     for (void *p : ptrs)
@@ -412,12 +335,11 @@ int check_err(moc_batch *b)
     return MOC_OK;
 }
 
-template <bool WRITE>
 void launch_btrace(moc_batch *b, int commit)
 {
     const int grid = (int)std::min(b->nst_all, (long long)(1 << 20));
 #define MOC_BT(K)                                                                              \
-    hipLaunchKernelGGL((k_btrace<WRITE, K>), dim3(grid), dim3(b->trace_threads), 0, b->stream, \
+    hipLaunchKernelGGL((k_btrace<K>), dim3(grid), dim3(b->trace_threads), 0, b->stream,       \
                        b->d_tr, b->d_stack_prob, b->d_stack_first, b->nst_all, commit)
     switch (b->kpt) {
     case 1: MOC_BT(1); break;
@@ -426,17 +348,6 @@ void launch_btrace(moc_batch *b, int commit)
     default: MOC_BT(8); break;
     }
 #undef MOC_BT
-}
-
-void launch_scan(moc_batch *b)
-{
-    const long long n = b->T_all + 1;
-    const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, b->stream, b->d_cnt,
-                       b->d_off, n, b->d_tile);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, b->stream, b->d_tile, tiles);
-    hipLaunchKernelGGL(k_scan_add, dim3(launch_grid(n, 256)), dim3(256), 0, b->stream, b->d_off, n,
-                       b->d_tile);
 }
 
 void launch_batt(moc_batch *b, const BatchItem *items, int n_items)
@@ -471,15 +382,13 @@ void enqueue_iteration(moc_batch *b)
     if (t64)
         hipLaunchKernelGGL(k_bseed, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
                            b->d_flux, b->d_acc, b->nval_all);
-    launch_btrace<false>(b, -1);
-    launch_scan(b);
-    launch_btrace<true>(b, -1);
+    launch_btrace(b, 1);
     launch_batt(b, b->d_items, b->n_items);
     if (t64)
         hipLaunchKernelGGL(k_bfinal, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
                            b->d_acc, b->d_flux, b->nval_all);
     hipLaunchKernelGGL(k_bstep, dim3(launch_grid(b->nprob, 64)), dim3(64), 0, b->stream, b->d_sweep,
-                       b->d_segs, b->d_off, b->d_track_first, b->nprob);
+                       b->nprob);
 }
 
 // Work items of the given problems: runs of tracks_per_item tracks (whole
@@ -710,7 +619,6 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         TRY(dalloc(&b->d_acc, b->nval_all));
     TRY(dalloc(&b->d_cnt, b->T_all + 1));
     TRY(dalloc(&b->d_off, b->T_all + 1));
-    TRY(dalloc(&b->d_tile, (b->T_all + 1 + SCAN_TILE - 1) / SCAN_TILE + 1));
     TRY(dalloc(&b->d_rec, b->rec_cap));
     TRY(dalloc(&b->d_sweep, n));
     TRY(dalloc(&b->d_segs, n));
@@ -723,6 +631,8 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
     TRY(dalloc(&b->d_err, 1));
 
     // uploads
+    std::vector<long long> off_host(b->T_all + 1);
+    long long rec_run = 0;
     std::vector<TraceArgs> tr(n);
     std::vector<AttArgs> at(n);
     std::vector<long long> track_first(n + 1), stack_first(n + 1);
@@ -785,6 +695,8 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         a.seed = q.seed;
         a.sweep_ctr = b->d_sweep + p;
         a.commit = 1;
+        a.seg_total = b->d_segs + p;
+        a.s_end = q.nst;
 
         AttArgs &t = at[p];
         memset(&t, 0, sizeof(t));
@@ -807,6 +719,17 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         t.dx = q.table_dx;
         t.hdx = 0.5f * q.table_dx;
         t.maxv = q.table_maxv;
+        t.cnt = b->d_cnt + q.track_first;
+        t.t_end = q.T;
+
+        // each track's record region: its stack's piece bound
+        std::vector<long long> per((size_t)q.nst);
+        TRY(moc_record_bounds(P, per.data(), nullptr));
+        for (long long st = 0; st < q.nst; st++)
+            for (int k = 0; k < q.zs; k++) {
+                off_host[q.track_first + st * q.zs + k] = rec_run;
+                rec_run += per[st];
+            }
 
         track_first[p] = q.track_first;
         stack_first[p] = q.stack_first;
@@ -814,6 +737,10 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
             stack_prob[q.stack_first + s] = p;
     }
     track_first[n] = b->T_all;
+    off_host[b->T_all] = rec_run;
+    if (rec_run != b->rec_cap)
+        return moc_set_error(MOC_ERR_STATE, "moc_batch_create: record regions disagree with the bound");
+    HIP_TRY(hipMemcpy(b->d_off, off_host.data(), (b->T_all + 1) * 8, hipMemcpyHostToDevice));
     stack_first[n] = b->nst_all;
     HIP_TRY(hipMemcpy(b->d_tr, tr.data(), n * sizeof(TraceArgs), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_at, at.data(), n * sizeof(AttArgs), hipMemcpyHostToDevice));
@@ -835,20 +762,17 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         HIP_TRY(hipGetLastError());
     }
 
-    // piece counts of the first sweep order the work queue (and feed the tuner)
+    // the first sweep's trace (z untouched): its piece counts order the
+    // work queue, its records feed the tuner
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
-    launch_btrace<false>(b, 0);
+    launch_btrace(b, 0);
     HIP_TRY(hipGetLastError());
     std::vector<int> cnt(b->T_all + 1);
     HIP_TRY(hipMemcpyAsync(cnt.data(), b->d_cnt, (b->T_all + 1) * 4, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    if (o->autotune) {
-        launch_scan(b);
-        launch_btrace<true>(b, 0);   // records of the first sweep, z untouched
-        HIP_TRY(hipGetLastError());
-        TRY(check_err(b));
+    TRY(check_err(b));
+    if (o->autotune)
         TRY(autotune(b, cnt));
-    }
     std::vector<int> all(n);
     for (int p = 0; p < n; p++)
         all[p] = p;

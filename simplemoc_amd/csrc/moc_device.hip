@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a)
 {
     __shared__ int s_w[16];
     for (long long s = a.s_begin + blockIdx.x; s < a.s_end; s += gridDim.x)
-        trace_stack<WRITE, KPT>(a, s, a.sweep_key, s_w);
+        trace_stack<WRITE ? TRACE_WRITE : TRACE_COUNT, KPT>(a, s, a.sweep_key, s_w);
 }
 
 // 256/LPT tracks per block

@@ -55,6 +55,7 @@ struct TraceArgs {
     const long long *sweep_ctr; // from (seed, *sweep_ctr)
     long long rec_base;         // record (and serial draw) index of rec[0]
     long long s_begin, s_end;   // stacks of this launch
+    long long *seg_total;       // TRACE_EMIT: pieces of the sweep are added here (or null)
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.
@@ -80,6 +81,7 @@ struct AttArgs {
     float maxv;
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch
+    const int *cnt;             // pieces of track t (null: off[t+1] - off[t])
 };
 
 }  // namespace moc

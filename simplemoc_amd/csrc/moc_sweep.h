@@ -256,11 +256,28 @@ __device__ __forceinline__ int block_min(int v, int *s_w)
     return m;
 }
 
+// What one trace of a z-stack produces.
+enum : int {
+    TRACE_COUNT = 0,   // pieces per track (and per stack)
+    TRACE_WRITE = 1,   // records at off[t]: each segment traced, then re-traced to emit
+    TRACE_EMIT = 2,    // records emitted while tracing, into per-track regions
+                       // [off[t], off[t+1]) sized by the piece bound; pieces per
+                       // track; no serial draw stream
+};
+
 // One z-stack s of the problem described by `a` (every thread of the
 // block calls it with the same s; s_w is 16 ints of LDS).
-template <bool WRITE, int KPT>
+//
+// TRACE_EMIT writes a track's records for a segment before it is known
+// whether the serial loop reaches that track in that segment; if it does
+// not, the track's count does not move and later segments write over
+// those records.  Every write stays inside the track's own region because
+// a segment traced at any z has at most its bound's pieces.
+template <int MODE, int KPT>
 __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uint64_t skey, int *s_w)
 {
+    constexpr bool WRITE = MODE == TRACE_WRITE;
+    constexpr bool EMIT = MODE == TRACE_EMIT;
     const int zs = a.zs;
     const int k0 = threadIdx.x * KPT;
     {
@@ -272,15 +289,16 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
         const double sn = a.sind[j];
         float z[KPT];
         int cnt[KPT];
-        long long base[KPT];
+        long long base[KPT], cap[KPT];
         uint64_t tkey[KPT];
 #pragma unroll
         for (int r = 0; r < KPT; r++) {
             const int k = k0 + r;
             z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
             cnt[r] = 0;
-            base[r] = (WRITE && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
-            tkey[r] = (WRITE && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
+            base[r] = ((WRITE || EMIT) && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
+            cap[r] = (EMIT && k < zs) ? a.off[t0 + k + 1] - a.rec_base : a.rec_cap;
+            tkey[r] = ((WRITE || EMIT) && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
         }
         int begin = 0, end = zs;
         long long serial = (WRITE && a.qsr_stream) ? a.stack_off[s] - a.rec_base : 0;
@@ -296,7 +314,21 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
             for (int r = 0; r < KPT; r++) {
                 const int k = k0 + r;
                 act[r] = (k >= begin && k < end);
-                if (act[r]) {
+                if (act[r] && EMIT) {
+                    EmitCtx ec;
+                    ec.rec = a.rec;
+                    ec.rec_cap = cap[r];
+                    ec.base = base[r] + cnt[r];
+                    ec.stream = nullptr;
+                    ec.serial = 0;
+                    ec.tkey = tkey[r];
+                    ec.nsr = a.nsr;
+                    ec.fai = a.fai;
+                    ec.quad = a.quad;
+                    ec.dz_att = a.dz_att;
+                    ec.err = a.err;
+                    res[r] = trace_segment<true>(z[r], s_full, k, pos, last, n, a, c, &ec);
+                } else if (act[r]) {
                     res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
                 } else {
                     res[r].pieces = 0;
@@ -380,14 +412,20 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
         for (int r = 0; r < KPT; r++) {
             const int k = k0 + r;
             if (k < zs) {
-                if (WRITE && a.commit)
+                if ((WRITE || EMIT) && a.commit)
                     a.z[t0 + k] = z[r];
-                else
+                if (EMIT || !(WRITE && a.commit))
                     a.cnt[t0 + k] = cnt[r];
             }
             stack_total += cnt[r];
         }
-        if (!WRITE && a.stack_cnt) {
+        if (EMIT && a.seg_total) {
+            int tot;
+            (void)block_excl_scan((int)stack_total, s_w, &tot);
+            if (threadIdx.x == 0)
+                atomicAdd((unsigned long long *)a.seg_total, (unsigned long long)tot);
+        }
+        if (MODE == TRACE_COUNT && a.stack_cnt) {
             int tot;
             (void)block_excl_scan((int)stack_total, s_w, &tot);
             if (threadIdx.x == 0)
@@ -457,7 +495,8 @@ __device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *
         for (int r = 0; r < GPL; r++)
             // a pending renormalisation (src/solver.c:1224) is applied here
             psi[r] = a.psi[t * G + gcl[r]] * pscale;
-        const long long q0 = a.off[t] - a.rec_base, q1 = a.off[t + 1] - a.rec_base;
+        const long long q0 = a.off[t] - a.rec_base;
+        const long long q1 = a.cnt ? q0 + a.cnt[t] : a.off[t + 1] - a.rec_base;
         for (long long qc = q0; qc < q1; qc += LPT) {
             const int nq = (int)min((long long)LPT, q1 - qc);
             const float4 mine = a.rec[qc + min(l, nq - 1)];
