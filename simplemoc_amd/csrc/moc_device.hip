@@ -166,6 +166,8 @@ struct AdjArgs {
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch
     const uint32_t *wmask;      // bit fsr: some group of FSR fsr has a nonzero weight
+    const int *order;           // tracks in processing order (t_end - t_begin of them)
+    const int *extent;          // [T] pieces up to and including the last weighted one
 };
 
 template <int EXPM>
@@ -214,7 +216,13 @@ struct PieceGrad {
 };
 
 // one piece backward: psp is the psi entering the piece, bw the tally's
-// weight in J and lam the adjoint of the psi leaving it
+// weight in J and lam the adjoint of the psi leaving it.  The terms of the
+// psi update (weighted by lam) are always formed; those of the tally
+// (weighted by bw, zero for most pieces of a sparse functional) only when
+// bw != 0, where the tally I itself (for J) is evaluated as the sweep
+// evaluates it.  The derivatives use the hardware reciprocal of sigma and
+// its powers instead of a division per term (a few ulp, far inside the
+// gradient tolerance).
 template <bool QUAD, int EXPM>
 __device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, float hdx,
                                                    float maxv, int N, float4 c, float ds, float zin,
@@ -226,47 +234,57 @@ __device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, 
     const float tau = sig * ds;
     float dF;
     const float F = exp_term_d<EXPM>(tab, tau, dx, hdx, maxv, N, &dF);
+    const float is = __builtin_amdgcn_rcpf(sig);
+    o.I = 0.f;
     if (QUAD) {
         const float q0v = c.x + c.y * zin + c.z * zin * zin;
         const float q1v = c.y + 2.f * c.z * zin;
         const float q2v = c.z;
-        const float s2 = sig * sig, s3 = s2 * sig, s4 = s2 * s2;
-        const float reuse = tau * (tau - 2.f) + 2.f * F / s3;
-        const float P3 = tau * (tau * (tau - 3.f) + 6.f) - 6.f * F;
-        const float A = (q0v * tau + (sig * psp - q0v) * F) / s2;
-        const float Cc = q2v * mu2 * P3 / (3.f * s4);
-        o.I = A + q1v * mu * reuse + Cc;
-        const float gq0 = bw * (tau - F) / s2 + lam * F / sig;
-        const float gq1 = bw * mu * reuse + lam * mu * (tau - F) / s2;
-        const float gq2 = bw * mu2 * P3 / (3.f * s4) + lam * mu2 * reuse;
+        const float is2 = is * is, is3 = is2 * is, is4 = is2 * is2;
+        const float TF = tau - F;
+        const float reuse = tau * (tau - 2.f) + 2.f * F * is3;
+        // psi update terms
+        float gq0 = lam * F * is;
+        float gq1 = lam * mu * TF * is2;
+        float gq2 = lam * mu2 * reuse;
+        const float dP_ds = -q0v * F * is2 - 2.f * q1v * mu * TF * is3 - 6.f * q2v * mu2 * F * is4;
+        const float dP_dt = q1v * mu * is2 + q2v * mu2 * (2.f * tau - 2.f);
+        const float dP_dF = q0v * is - q1v * mu * is2 + 2.f * q2v * mu2 * is3 - psp;
+        o.gs = lam * (dP_ds + (dP_dt + dP_dF * dF) * ds);
+        o.lam = lam * (1.f - F);
+        if (bw != 0.f) {
+            // tally terms
+            const float P3 = tau * (tau * (tau - 3.f) + 6.f) - 6.f * F;
+            const float third_is4 = is4 * (1.0f / 3.0f);
+            const float s2 = sig * sig;
+            const float reuse_s = tau * (tau - 2.f) + 2.f * F / (sig * s2);
+            o.I = (q0v * tau + (sig * psp - q0v) * F) / s2 + q1v * mu * reuse_s +
+                  q2v * mu2 * P3 / (3.f * s2 * s2);
+            gq0 += bw * TF * is2;
+            gq1 += bw * mu * reuse;
+            gq2 += bw * mu2 * P3 * third_is4;
+            const float A = (q0v * tau + (sig * psp - q0v) * F) * is2;
+            const float Cc = q2v * mu2 * P3 * third_is4;
+            const float dI_ds = psp * F * is2 - 2.f * A * is - 6.f * q1v * mu * F * is4 - 4.f * Cc * is;
+            const float dI_dt = q0v * is2 + q1v * mu * (2.f * tau - 2.f) +
+                                q2v * mu2 * (3.f * tau * tau - 6.f * tau + 6.f) * third_is4;
+            const float dI_dF = (sig * psp - q0v) * is2 + 2.f * q1v * mu * is3 - 2.f * q2v * mu2 * is4;
+            o.gs += bw * (dI_ds + (dI_dt + dI_dF * dF) * ds);
+            o.lam += bw * F * is;
+        }
         o.gc0 = gq0;
         o.gc1 = gq0 * zin + gq1;
         o.gc2 = gq0 * zin * zin + 2.f * gq1 * zin + gq2;
-        const float dI_ds = psp * F / s2 - 2.f * A / sig - 6.f * q1v * mu * F / s4 - 4.f * Cc / sig;
-        const float dI_dt = q0v / s2 + q1v * mu * (2.f * tau - 2.f) +
-                            q2v * mu2 * (3.f * tau * tau - 6.f * tau + 6.f) / (3.f * s4);
-        const float dI_dF = (sig * psp - q0v) / s2 + 2.f * q1v * mu / s3 - 2.f * q2v * mu2 / s4;
-        const float dP_ds = -q0v * F / s2 - 2.f * q1v * mu * (tau - F) / s3 - 6.f * q2v * mu2 * F / s4;
-        const float dP_dt = q1v * mu / s2 + q2v * mu2 * (2.f * tau - 2.f);
-        const float dP_dF = q0v / sig - q1v * mu / s2 + 2.f * q2v * mu2 / s3 - psp;
-        o.gs = bw * (dI_ds + (dI_dt + dI_dF * dF) * ds) + lam * (dP_ds + (dP_dt + dP_dF * dF) * ds);
-        o.lam = bw * F / sig + lam * (1.f - F);
     } else {
         const float qv = c.x / sig;
         o.I = (psp - qv) * F;
-        o.gc0 = (lam - bw) * F / sig;
+        o.gc0 = (lam - bw) * F * is;
         o.gc1 = 0.f;
         o.gc2 = 0.f;
-        o.gs = (bw - lam) * (qv * F / sig + (psp - qv) * dF * ds);
+        o.gs = (bw - lam) * (qv * F * is + (psp - qv) * dF * ds);
         o.lam = bw * F + lam * (1.f - F);
     }
     return o;
-}
-
-__device__ __forceinline__ void add_nonzero(double *p, float v)
-{
-    if (v != 0.0f)
-        atomicAdd(p, (double)v);
 }
 
 // lane l of a track owns groups l + r*LPT, r < GPL; PB pieces' loads
@@ -284,6 +302,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
     const int wv = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
     const int l = threadIdx.x % LPT;
+    const int lane_base = lane - l;
     const int slot = (LPT == 64) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64))
                                  : (int)(threadIdx.x / LPT);
     const int G = a.G;
@@ -301,110 +320,149 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
     double Jacc = 0.0;
     const long long nblk = (a.t_end - a.t_begin + TPB - 1) / TPB;
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const long long t = a.t_begin + b * TPB + slot;
-        const bool has = t < a.t_end;
-        const long long tt = has ? t : a.t_begin;
+        const long long u = b * TPB + slot;
+        const bool has = u < a.t_end - a.t_begin;
+        const long long tt = has ? (long long)a.order[u] : a.t_begin;
         const int j = (int)((tt / a.zs) % a.np);
         const long long i = tt / ((long long)a.zs * a.np);
         const float mu = a.mu[j];
         const float mu2 = mu * mu;
         const float wt = QUAD ? a.pw[tt] * a.az[i] : a.pw[tt] * a.az[i] * mu;
         const long long q0 = has ? a.off[tt] - a.rec_base : 0;
-        const int P_all = has ? (int)(a.off[tt + 1] - a.off[tt]) : 0;
         // Pieces after the track's last weighted one carry no weight and a
         // zero adjoint, so they add nothing to J or to any gradient: the
-        // track is processed up to that piece only.  The track's lanes scan
-        // its records LPT at a time against the FSR weight mask.
-        int last = -1;
-        for (int p0 = 0; p0 < P_all; p0 += LPT) {
-            const int p = p0 + l;
-            if (p < P_all) {
-                const uint32_t f = __float_as_uint(a.rec[q0 + p].z);
-                if ((a.wmask[f >> 5] >> (f & 31)) & 1u)
-                    last = p;
-            }
-        }
-#pragma unroll
-        for (int o = 1; o < LPT; o <<= 1)
-            last = max(last, __shfl_xor(last, o, 64));
-        const int P = last + 1;
+        // track is processed up to that piece only (its extent, from the
+        // record write; tracks come longest first so that the tracks
+        // sharing a wavefront have similar extents).  Lane l holds record
+        // c0 + l of the current run of LPT records, broadcast lane to lane.
+        const int P = has ? a.extent[tt] : 0;
         float ps[GPL];
 #pragma unroll
         for (int r = 0; r < GPL; r++)
             ps[r] = a.psi[tt * G + gg[r]] * pscale;
         // forward: psi entering each piece onto the tape
-        for (int p0 = 0; p0 < P; p0 += PB) {
-            float4 rc[PB], c[PB][GPL];
+        for (int c0 = 0; c0 < P; c0 += LPT) {
+            const float4 mine = a.rec[q0 + min(c0 + l, P - 1)];
+            // the run's records are waited for here, once, not inside the
+            // loop (where the wait would also drain the stores and atomics)
+            asm volatile("" ::"v"(mine.x), "v"(mine.y), "v"(mine.z));
+            const int nq = min(LPT, P - c0);
+            for (int u0 = 0; u0 < nq; u0 += PB) {
+                float ds[PB], zin[PB];
+                uint32_t fsr[PB];
+                float4 c[PB][GPL];
 #pragma unroll
-            for (int k = 0; k < PB; k++)
-                rc[k] = a.rec[q0 + min(p0 + k, P - 1)];
+                for (int k = 0; k < PB; k++) {
+                    const int pu = min(u0 + k, nq - 1);
+                    ds[k] = bcast<LPT>(mine.x, pu, lane_base);
+                    zin[k] = bcast<LPT>(mine.y, pu, lane_base);
+                    fsr[k] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
 #pragma unroll
-            for (int k = 0; k < PB; k++)
+                    for (int r = 0; r < GPL; r++)
+                        c[k][r] = a.coef[(size_t)fsr[k] * G + gg[r]];
+                }
 #pragma unroll
-                for (int r = 0; r < GPL; r++)
-                    c[k][r] = a.coef[(size_t)__float_as_uint(rc[k].z) * G + gg[r]];
+                for (int k = 0; k < PB; k++) {
+                    if (u0 + k < nq) {
 #pragma unroll
-            for (int k = 0; k < PB; k++) {
-                if (p0 + k < P) {
-#pragma unroll
-                    for (int r = 0; r < GPL; r++) {
-                        if (live[r])
-                            tape[(long long)(p0 + k) * G + gg[r]] = ps[r];
-                        piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k][r], rc[k].x, rc[k].y,
-                                                  mu, mu2, ps[r]);
+                        for (int r = 0; r < GPL; r++) {
+                            if (live[r])
+                                tape[(long long)(c0 + u0 + k) * G + gg[r]] = ps[r];
+                            piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k][r], ds[k],
+                                                      zin[k], mu, mu2, ps[r]);
+                        }
                     }
                 }
             }
         }
-        // reverse
+        // reverse, run by run, batch by batch: the atomics of a batch are
+        // issued after the next batch's loads, so the wait for those loads
+        // does not wait for them, and they drain while the next batch is
+        // computed (every atomic is issued, adding 0.0 where there is
+        // nothing: a fixed count keeps that wait exact)
         float lam[GPL];
 #pragma unroll
         for (int r = 0; r < GPL; r++)
             lam[r] = 0.f;
-        for (int p1 = P - 1; p1 >= 0; p1 -= PB) {
-            float4 rc[PB], c[PB][GPL];
-            float wv_[PB][GPL], psp[PB][GPL];
+        for (int c0 = P > 0 ? ((P - 1) / LPT) * LPT : -1; c0 >= 0; c0 -= LPT) {
+            const float4 mine = a.rec[q0 + min(c0 + l, P - 1)];
+            asm volatile("" ::"v"(mine.x), "v"(mine.y), "v"(mine.z));
+            const int nq = min(LPT, P - c0);
+            float ds[PB], zin[PB], wv_[PB][GPL], psp[PB][GPL];
+            uint32_t fsr[PB];
+            float4 c[PB][GPL];
+            // batch ending at run piece u1 (pieces u1, u1-1, ...)
+            auto load = [&](int u1) {
 #pragma unroll
-            for (int k = 0; k < PB; k++)
-                rc[k] = a.rec[q0 + max(p1 - k, 0)];
-#pragma unroll
-            for (int k = 0; k < PB; k++) {
-                // weights are gathered only for FSRs that have some (the
-                // same for all lanes of the track)
-                const uint32_t f = __float_as_uint(rc[k].z);
-                const bool weighted = (a.wmask[f >> 5] >> (f & 31)) & 1u;
-#pragma unroll
-                for (int r = 0; r < GPL; r++) {
-                    const size_t e = (size_t)f * G + gg[r];
-                    c[k][r] = a.coef[e];
-                    wv_[k][r] = weighted ? a.w[e] : 0.0f;
-                    psp[k][r] = tape[(long long)max(p1 - k, 0) * G + gg[r]];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < PB; k++) {
-                if (p1 - k >= 0) {
-                    const uint32_t fsr = __float_as_uint(rc[k].z);
+                for (int k = 0; k < PB; k++) {
+                    const int pu = max(u1 - k, 0);
+                    ds[k] = bcast<LPT>(mine.x, pu, lane_base);
+                    zin[k] = bcast<LPT>(mine.y, pu, lane_base);
+                    fsr[k] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
 #pragma unroll
                     for (int r = 0; r < GPL; r++) {
-                        const float bw = wv_[k][r] * wt;
-                        const PieceGrad o = piece_reverse<QUAD, EXPM>(
-                            s_tab, dx, hdx, maxv, N, c[k][r], rc[k].x, rc[k].y, mu, mu2, psp[k][r],
-                            bw, lam[r]);
-                        if (live[r]) {
-                            Jacc += (double)bw * (double)o.I;
-                            double *ap = a.acc + (size_t)fsr * 4 * G + gg[r];
-                            add_nonzero(ap, o.gc0);
-                            if (QUAD) {
-                                add_nonzero(ap + G, o.gc1);
-                                add_nonzero(ap + 2 * G, o.gc2);
-                            }
-                            add_nonzero(ap + 3 * G, o.gs);
-                        }
-                        lam[r] = o.lam;
+                        const size_t e = (size_t)fsr[k] * G + gg[r];
+                        c[k][r] = a.coef[e];
+                        wv_[k][r] = a.w[e];
+                        psp[k][r] = tape[(long long)(c0 + pu) * G + gg[r]];
                     }
                 }
+            };
+            float g0[PB][GPL], g1[PB][GPL], g2[PB][GPL], gsg[PB][GPL];
+            uint32_t gf[PB];
+            auto compute = [&](int u1) {
+#pragma unroll
+                for (int k = 0; k < PB; k++) {
+                    gf[k] = fsr[k];
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        g0[k][r] = g1[k][r] = g2[k][r] = gsg[k][r] = 0.f;
+                        if (u1 - k >= 0) {
+                            const float bw = wv_[k][r] * wt;
+                            const PieceGrad o = piece_reverse<QUAD, EXPM>(
+                                s_tab, dx, hdx, maxv, N, c[k][r], ds[k], zin[k], mu, mu2,
+                                psp[k][r], bw, lam[r]);
+                            if (live[r]) {
+                                if (bw != 0.f)
+                                    Jacc += (double)bw * (double)o.I;
+                                g0[k][r] = o.gc0;
+                                g1[k][r] = o.gc1;
+                                g2[k][r] = o.gc2;
+                                gsg[k][r] = o.gs;
+                            }
+                            lam[r] = o.lam;
+                        }
+                    }
+                }
+            };
+            // valid = pieces of the batch that exist (the batch is full
+            // except the last of a run)
+            auto flush = [&](int valid) {
+#pragma unroll
+                for (int k = 0; k < PB; k++)
+#pragma unroll
+                    for (int r = 0; r < GPL; r++) {
+                        if (k >= valid)
+                            continue;
+                        double *ap = a.acc + (size_t)gf[k] * 4 * G + gg[r];
+                        atomicAdd(ap, (double)g0[k][r]);
+                        if (QUAD) {
+                            atomicAdd(ap + G, (double)g1[k][r]);
+                            atomicAdd(ap + 2 * G, (double)g2[k][r]);
+                        }
+                        atomicAdd(ap + 3 * G, (double)gsg[k][r]);
+                    }
+            };
+            load(nq - 1);
+            compute(nq - 1);
+            int u_last = nq - 1;
+            for (int u1 = nq - 1 - PB; u1 >= 0; u1 -= PB) {
+                load(u1);
+                flush(PB);
+                compute(u1);
+                u_last = u1;
             }
+            flush(min(PB, u_last + 1));
         }
     }
     // J: wave then block sum, one atomic per block
@@ -821,6 +879,10 @@ struct moc_ctx {
     double *d_adj_gs = nullptr, *d_adj_gc = nullptr, *d_adj_J = nullptr, *d_adj_acc = nullptr;
     int *d_adj_maxp = nullptr;
     uint32_t *d_adj_mask = nullptr;
+    int *d_adj_ext = nullptr, *d_adj_key = nullptr, *d_adj_key2 = nullptr, *d_adj_id = nullptr,
+        *d_adj_order = nullptr;
+    void *d_adj_sort_tmp = nullptr;
+    size_t adj_sort_bytes = 0;
     double *d_adj_rtmp = nullptr;
     size_t adj_rtmp_bytes = 0;
     float *d_adj_tape = nullptr;
@@ -878,7 +940,9 @@ static void free_all(moc_ctx *c)
                     c->d_stream, c->d_scan_tmp, c->d_partial,
                     c->d_gres, c->d_rres, c->d_scal, c->d_err, c->d_user_c, c->d_user_s,
                     c->d_adj_w, c->d_adj_gs, c->d_adj_gc, c->d_adj_J, c->d_adj_acc,
-                    c->d_adj_maxp, c->d_adj_rtmp, c->d_adj_tape, c->d_adj_mask};
+                    c->d_adj_maxp, c->d_adj_rtmp, c->d_adj_tape, c->d_adj_mask,
+                    c->d_adj_ext, c->d_adj_key, c->d_adj_key2, c->d_adj_id, c->d_adj_order,
+                    c->d_adj_sort_tmp};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1421,7 +1485,7 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
 // rand() draws, the chunk's draws come next in the serial order
 // (src/solver.c:481: stack-major, then the trace order), which is the
 // chunk's record range.
-static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit)
+static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit, const uint32_t *wmask = nullptr)
 {
     TraceArgs ta = trace_args(c);
     ta.commit = commit ? 1 : 0;
@@ -1430,6 +1494,14 @@ static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit)
     ta.rec_base = ch.rec_base;
     ta.s_begin = ch.s0;
     ta.s_end = ch.s1;
+    if (wmask) {
+        // the adjoint: extents of the chunk's tracks and their sort input
+        ta.wmask = wmask;
+        ta.extent = c->d_adj_ext;
+        ta.sort_key = c->d_adj_key;
+        ta.sort_id = c->d_adj_id;
+        ta.key_base = ch.s0 * c->zs;
+    }
     if (c->opt.qsr_mode == MOC_QSR_LIBC) {
         if (ch.S > c->stream_cap) {
             HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1966,6 +2038,16 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
         TRY(dalloc(&c->d_adj_J, 1));
         TRY(dalloc(&c->d_adj_maxp, 1));
         TRY(dalloc(&c->d_adj_mask, (size_t)(c->nfsr + 31) / 32));
+        TRY(dalloc(&c->d_adj_ext, (size_t)c->T));
+        TRY(dalloc(&c->d_adj_key, (size_t)c->T));
+        TRY(dalloc(&c->d_adj_key2, (size_t)c->T));
+        TRY(dalloc(&c->d_adj_id, (size_t)c->T));
+        TRY(dalloc(&c->d_adj_order, (size_t)c->T));
+        size_t sb = 0;
+        HIP_TRY(rocprim::radix_sort_pairs_desc(nullptr, sb, c->d_adj_key, c->d_adj_key2, c->d_adj_id,
+                                               c->d_adj_order, (size_t)c->T, 0, 32, c->stream));
+        HIP_TRY(hipMalloc(&c->d_adj_sort_tmp, sb ? sb : 1));
+        c->adj_sort_bytes = sb;
         size_t rb = 0;
         HIP_TRY(rocprim::reduce(nullptr, rb, c->d_cnt, c->d_adj_maxp, 0, (size_t)c->T,
                                 rocprim::maximum<int>(), c->stream));
@@ -1986,7 +2068,10 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     if (maxp < 1)
         maxp = 1;
     const int tpb = 64 * ADJ_WAVES / c->lpt;
-    const long long nblk = (c->T + tpb - 1) / tpb;
+    long long max_chunk_tracks = 0;
+    for (const Chunk &ch : chunks)
+        max_chunk_tracks = std::max(max_chunk_tracks, (ch.s1 - ch.s0) * (long long)c->zs);
+    const long long nblk = (max_chunk_tracks + tpb - 1) / tpb;
     const long long per_block = (long long)tpb * maxp * c->G;   // tape floats per block
     long long grid = nblk < ADJ_MAX_GRID ? nblk : ADJ_MAX_GRID;
     if (grid * per_block > ADJ_TAPE_FLOATS)
@@ -2034,13 +2119,26 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     const size_t lds = (size_t)c->table_N * sizeof(float2);
     float ms_k = 0.f;
     for (const Chunk &ch : chunks) {
-        TRY(write_chunk(c, ch, false));
+        TRY(write_chunk(c, ch, false, c->d_adj_mask));
         note_chunk(c, ch);
         aa.rec = c->d_rec;
         aa.rec_base = ch.rec_base;
         aa.t_begin = ch.s0 * c->zs;
         aa.t_end = ch.s1 * c->zs;
+        const long long nt = aa.t_end - aa.t_begin;
         HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+        // tracks longest first (extents from the record write)
+        {
+            int bits = 1;
+            while (bits < 31 && (1LL << bits) <= (long long)maxp)
+                bits++;
+            size_t sb = c->adj_sort_bytes;
+            HIP_TRY(rocprim::radix_sort_pairs_desc(c->d_adj_sort_tmp, sb, c->d_adj_key, c->d_adj_key2,
+                                                   c->d_adj_id, c->d_adj_order, (size_t)nt, 0, bits,
+                                                   c->stream));
+        }
+        aa.order = c->d_adj_order;
+        aa.extent = c->d_adj_ext;
         switch (c->lpt) {
         case 8: launch_adj<8, 1>(c, aa, (int)grid, lds); break;
         case 16: launch_adj<16, 1>(c, aa, (int)grid, lds); break;

@@ -56,6 +56,14 @@ struct TraceArgs {
     long long rec_base;         // record (and serial draw) index of rec[0]
     long long s_begin, s_end;   // stacks of this launch
     long long *seg_total;       // TRACE_EMIT: pieces of the sweep are added here (or null)
+    // optional, record-writing traces: per track the pieces up to its last
+    // one in an FSR marked in wmask, and (key, id) = (that count, track) at
+    // index t - key_base for a sort by it
+    const uint32_t *wmask;
+    int *extent;
+    int *sort_key;
+    int *sort_id;
+    long long key_base;
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.

@@ -118,6 +118,7 @@ struct SegResult {
     int pieces;
     int exited;
     float zfin;
+    int last_weighted;          // emitting with a weight mask: last piece in a marked FSR (-1: none)
 };
 
 struct EmitCtx {
@@ -132,6 +133,7 @@ struct EmitCtx {
     int quad;
     float dz_att;
     unsigned *err;
+    const uint32_t *wmask;      // FSRs to watch for SegResult::last_weighted (or null)
 };
 
 // One (track, 2D segment): the while loop of src/solver.c:409-525.
@@ -145,6 +147,7 @@ __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k
     bool complete = false;
     int p = 0;
     int exited = 0;
+    int last_w = -1;
     while (!complete) {
         bool reset = false;
         float ds;
@@ -193,10 +196,13 @@ __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k
                 fine = 0;
             }
             const long long q = ec->base + p;
+            const uint32_t fsr = qsr * (uint32_t)ec->fai + (uint32_t)fine;
             if (q < ec->rec_cap)
-                ec->rec[q] = make_float4(ds, zin, __uint_as_float(qsr * (uint32_t)ec->fai + (uint32_t)fine), 0.f);
+                ec->rec[q] = make_float4(ds, zin, __uint_as_float(fsr), 0.f);
             else
                 atomicOr(ec->err, ERR_RECORD_CAP);
+            if (ec->wmask && ((ec->wmask[fsr >> 5] >> (fsr & 31)) & 1u))
+                last_w = p;
         }
         p++;
         if (p >= MAX_PIECES_PER_SEGMENT && !complete) {
@@ -212,6 +218,7 @@ __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k
     r.pieces = p;
     r.exited = exited;
     r.zfin = zk;
+    r.last_weighted = last_w;
     return r;
 }
 
@@ -290,12 +297,14 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
         float z[KPT];
         int cnt[KPT];
         long long base[KPT], cap[KPT];
+        int lastw[KPT];
         uint64_t tkey[KPT];
 #pragma unroll
         for (int r = 0; r < KPT; r++) {
             const int k = k0 + r;
             z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
             cnt[r] = 0;
+            lastw[r] = -1;
             base[r] = ((WRITE || EMIT) && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
             cap[r] = (EMIT && k < zs) ? a.off[t0 + k + 1] - a.rec_base : a.rec_cap;
             tkey[r] = ((WRITE || EMIT) && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
@@ -327,6 +336,7 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
                     ec.quad = a.quad;
                     ec.dz_att = a.dz_att;
                     ec.err = a.err;
+                    ec.wmask = a.wmask;
                     res[r] = trace_segment<true>(z[r], s_full, k, pos, last, n, a, c, &ec);
                 } else if (act[r]) {
                     res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
@@ -400,9 +410,14 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
                     ec.quad = a.quad;
                     ec.dz_att = a.dz_att;
                     ec.err = a.err;
-                    (void)trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
+                    ec.wmask = a.wmask;
+                    const SegResult e = trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
+                    if (e.last_weighted >= 0)
+                        lastw[r] = cnt[r] + e.last_weighted;
                     ser += res[r].pieces;
                 }
+                if (EMIT && res[r].last_weighted >= 0)
+                    lastw[r] = cnt[r] + res[r].last_weighted;
                 cnt[r] += res[r].pieces;
                 z[r] = res[r].zfin;
             }
@@ -416,6 +431,14 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
                     a.z[t0 + k] = z[r];
                 if (EMIT || !(WRITE && a.commit))
                     a.cnt[t0 + k] = cnt[r];
+                // with a weight mask: each track's pieces up to its last
+                // weighted one, and the input of the sort by that extent
+                if ((WRITE || EMIT) && a.extent) {
+                    const long long t = t0 + k;
+                    a.extent[t] = lastw[r] + 1;
+                    a.sort_key[t - a.key_base] = lastw[r] + 1;
+                    a.sort_id[t - a.key_base] = (int)t;
+                }
             }
             stack_total += cnt[r];
         }
