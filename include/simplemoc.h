@@ -9,6 +9,7 @@
  * (SimpleMOC_header.h:225-237, called from src/main.c:61,75,81,87):
  *
  *   moc_transport_sweep   <-> transport_sweep(Params*, Input*)     src/solver.c:283
+ *   moc_two_way_sweep     <-> two_way_transport_sweep(Params*, Input*) src/solver.c:556
  *   moc_boundary_exchange <-> fast_transfer_boundary_fluxes(...)   src/comms.c:5
  *   moc_renormalize_flux  <-> renormalize_flux(Params, Input, CommGrid) src/solver.c:1143
  *   moc_update_sources    <-> update_sources(Params, Input, float)  src/solver.c:1235
@@ -170,6 +171,13 @@ int moc_destroy(moc_ctx *ctx);
 
 /* src/solver.c:283-552.  Synchronous.  *segments_processed may be NULL. */
 int moc_transport_sweep(moc_ctx *ctx, long *segments_processed);
+/* src/solver.c:556-891: every track traced forward (f_psi, mu), then its
+ * pieces replayed backwards (b_psi, -mu); z heights end at their starting
+ * values.  The device keeps b_psi from the first call on (zero, or NaN
+ * after a non-finite normalisation factor, as the host copy would be).
+ * Counts 2 x pieces for axial_exp 2 and 0 for axial_exp 0, as the
+ * reference does.  Piece records take two slots per piece. */
+int moc_two_way_sweep(moc_ctx *ctx, long *segments_processed);
 /* src/comms.c:5-196 for one domain: no-op (reflective) or vacuum tally. */
 int moc_boundary_exchange(moc_ctx *ctx);
 /* src/solver.c:1143-1230, FP64 fission-rate reduction. */

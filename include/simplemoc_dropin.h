@@ -8,6 +8,8 @@
  * layouts and by-value argument passing:
  *
  *   transport_sweep   replaces src/solver.c:283  (declared SimpleMOC_header.h:225)
+ *   two_way_transport_sweep replaces src/solver.c:556 (not declared in the
+ *                     reference's header; same signature as transport_sweep)
  *   renormalize_flux  replaces src/solver.c:1143 (declared SimpleMOC_header.h:235)
  *   update_sources    replaces src/solver.c:1235 (declared SimpleMOC_header.h:236)
  *   compute_keff      replaces src/solver.c:1324 (declared SimpleMOC_header.h:237)
@@ -142,6 +144,7 @@ typedef struct {
 } moc_ref_CommGrid;
 
 void transport_sweep(moc_ref_Params *params, moc_ref_Input *I);
+void two_way_transport_sweep(moc_ref_Params *params, moc_ref_Input *I);
 void renormalize_flux(moc_ref_Params params, moc_ref_Input I, moc_ref_CommGrid grid);
 float update_sources(moc_ref_Params params, moc_ref_Input I, float keff);
 float compute_keff(moc_ref_Params params, moc_ref_Input I, moc_ref_CommGrid grid);

@@ -403,6 +403,165 @@ long orc_transport_sweep(orc_problem *P, orc_opts *o)
 }
 
 /* ------------------------------------------------------------------ */
+/* src/solver.c:556-891: the two-way sweep.  Per 2D track and polar angle:
+ * the stacked tracks are traced forward segment by segment (the interval
+ * and boundary arithmetic of :650-755, each piece attenuated into f_psi as
+ * it is traced, its length and region kept), the stack bounds shrinking by
+ * the number of tracks that reached the boundary after each segment's
+ * k-loop (:805-808); then every stacked track replays its pieces in
+ * reverse into b_psi with -mu, z decremented by ds*mu after each
+ * (:811-842); then all z return to their starting heights (:846-853).
+ * Only quadratic-source pieces are counted (:782, :827).  Returns the
+ * count, or -1. */
+long orc_two_way_sweep(orc_problem *P, orc_opts *o)
+{
+    if (P->axial_exp != 0 && P->axial_exp != 2)
+        return -1;
+    if (!P->b_psi)
+        return -1;
+    const int G = P->G;
+    const long nval = P->nsr * P->fai * G;
+    double *acc = NULL;
+    if (o->tally == ORC_TALLY_F64) {
+        acc = (double *)malloc((size_t)nval * sizeof(double));
+        for (long e = 0; e < nval; e++)
+            acc[e] = (double)P->fine_flux[e];
+    }
+    const double node_dz = (double)(P->height / (float)P->decomp);        /* :560 */
+    const int num_intervals = P->cai * P->fai;
+    const double fine_dz = node_dz / num_intervals;
+    const int zs = P->z_stacked;
+    long cap = 64;
+    float **dist = (float **)malloc(zs * sizeof(float *));
+    long **src = (long **)malloc(zs * sizeof(long *));
+    long *cnt = (long *)calloc(zs, sizeof(long));
+    long *size = (long *)malloc(zs * sizeof(long));
+    for (int k = 0; k < zs; k++) {
+        size[k] = cap;
+        dist[k] = (float *)malloc(cap * sizeof(float));
+        src[k] = (long *)malloc(cap * sizeof(long));
+    }
+    long total = 0;
+    int failed = 0;
+    for (long i = 0; i < P->ntracks_2D && !failed; i++) {
+        const float azw = P->az_weight[i];
+        const long off = P->seg_offset[i];
+        const long nseg = P->seg_offset[i + 1] - off;
+        for (int j = 0; j < P->n_polar && !failed; j++) {
+            const int pos = j < P->n_polar / 2;                            /* :634-635 */
+            const float p_angle = P->polar_angle[j];
+            const float mu = (float)cos((double)p_angle);
+            float *z = P->z_height + ((long)i * P->n_polar + j) * zs;
+            const long t0 = ((long)i * P->n_polar + j) * zs;
+            int begin = 0, end = zs;
+            for (int k = 0; k < zs; k++)
+                cnt[k] = 0;
+            for (long n = 0; n < nseg && !failed; n++) {
+                const float s_full = (float)(P->seg_length[off + n] / sin((double)p_angle));
+                int completed = 0;
+                for (int k = begin; k < end; k++) {
+                    const long t = t0 + k;
+                    /* :671 -- the cast binds to z_height alone */
+                    int interval = (int)((double)(int)z[k] / fine_dz);
+                    float bound;
+                    if (pos)
+                        bound = (float)((node_dz - (double)z[k]) / (double)mu);
+                    else
+                        bound = -z[k] / mu;
+                    float s;
+                    if (s_full < bound) {
+                        s = s_full;
+                    } else {
+                        s = bound;
+                        completed++;
+                    }
+                    int done = 0;
+                    int p = 0;
+                    while (!done) {
+                        float fai_z = pos ? (float)((interval + 1) * fine_dz)
+                                          : (float)(interval * fine_dz);
+                        float zd = fai_z - z[k];
+                        float sd = zd / mu;
+                        float ds, zn;
+                        if (sd < s) {
+                            interval += pos ? 1 : -1;
+                            ds = sd;
+                            zn = z[k] + zd;
+                        } else {
+                            ds = s;
+                            zn = z[k] + s * mu;
+                        }
+                        s -= ds;
+                        if (s <= 0 || interval < 0 || interval >= num_intervals)
+                            done = 1;
+                        long qsr;
+                        if (o->qsr_mode == ORC_QSR_LIBC) {
+                            qsr = rand() % P->nsr;
+                        } else if (o->qsr_mode == ORC_QSR_STREAM) {
+                            if (!o->qsr_stream || o->qsr_stream_pos >= o->qsr_stream_len) {
+                                failed = 1;
+                                break;
+                            }
+                            qsr = o->qsr_stream[o->qsr_stream_pos++];
+                        } else {
+                            qsr = (long)orc_qsr_hash(o->seed, o->sweep_index, t, n, p, P->nsr);
+                        }
+                        orc_attenuate(P, o, t, z[k], ds, qsr, mu, azw, P->f_psi + t * G,
+                                      P->fine_flux, acc);
+                        if (P->axial_exp == 2)
+                            total++;
+                        z[k] = zn;
+                        if (cnt[k] >= size[k]) {
+                            size[k] *= 2;
+                            dist[k] = (float *)realloc(dist[k], size[k] * sizeof(float));
+                            src[k] = (long *)realloc(src[k], size[k] * sizeof(long));
+                        }
+                        dist[k][cnt[k]] = ds;
+                        src[k][cnt[k]] = qsr;
+                        cnt[k]++;
+                        p++;
+                    }
+                    if (failed)
+                        break;
+                }
+                if (pos)
+                    end -= completed;
+                else
+                    begin += completed;
+            }
+            for (int k = 0; k < zs && !failed; k++) {
+                const long t = t0 + k;
+                for (long q = cnt[k] - 1; q >= 0; q--) {
+                    orc_attenuate(P, o, t, z[k], dist[k][q], src[k][q], -mu, azw,
+                                  P->b_psi + t * G, P->fine_flux, acc);
+                    if (P->axial_exp == 2)
+                        total++;
+                    z[k] -= dist[k][q] * mu;
+                }
+            }
+            for (int k = 0; k < zs; k++)
+                z[k] = pos ? P->axial_z_sep * (float)k : P->axial_z_sep * (float)(k + 1);
+        }
+    }
+    for (int k = 0; k < zs; k++) {
+        free(dist[k]);
+        free(src[k]);
+    }
+    free(dist);
+    free(src);
+    free(cnt);
+    free(size);
+    if (acc) {
+        if (!failed)
+            for (long e = 0; e < nval; e++)
+                P->fine_flux[e] = (float)acc[e];
+        free(acc);
+    }
+    o->sweep_index++;
+    return failed ? -1 : total;
+}
+
+/* ------------------------------------------------------------------ */
 /* src/comms.c:5-196 on a domain whose six neighbours are all absent: every
  * face chunk is a border chunk, so its pairwise sum goes to leakage and
  * the received buffer (zeros) is copied back over it.  The flat flux

@@ -15,6 +15,7 @@
  * Reference files restated (paths relative to the reference root):
  *   src/solver.c:14-280     attenuate_fluxes (quadratic source)
  *   src/solver.c:283-552    transport_sweep (axial on-the-fly ray trace)
+ *   src/solver.c:556-891    two_way_transport_sweep
  *   src/solver.c:895-907    get_pos_interval / get_neg_interval
  *   src/solver.c:1040-1138  attenuate_FSR_fluxes (flat source)
  *   src/solver.c:1143-1230  renormalize_flux (serial build semantics)
@@ -166,6 +167,8 @@ void orc_attenuate(const orc_problem *P, const orc_opts *o, long track,
 
 /* ---- per-iteration path ---- */
 long orc_transport_sweep(orc_problem *P, orc_opts *o);
+/* src/solver.c:556-891 (forward into f_psi, backward into b_psi) */
+long orc_two_way_sweep(orc_problem *P, orc_opts *o);
 void orc_boundary_vacuum(orc_problem *P, const orc_opts *o);
 void orc_renormalize_flux(orc_problem *P, const orc_opts *o);
 float orc_update_sources(orc_problem *P, float keff);

@@ -99,6 +99,8 @@ def lib():
                                     _fp, _fp, C.POINTER(C.c_double)]
         L.orc_transport_sweep.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_transport_sweep.restype = C.c_long
+        L.orc_two_way_sweep.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
+        L.orc_two_way_sweep.restype = C.c_long
         L.orc_boundary_vacuum.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_renormalize_flux.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_update_sources.argtypes = [C.POINTER(OrcProblem), C.c_float]
@@ -214,6 +216,13 @@ class Oracle:
             raise RuntimeError("oracle sweep failed")
         return int(n)
 
+    def two_way_sweep(self) -> int:
+        """src/solver.c:556-891 (needs a b_psi array)."""
+        n = lib().orc_two_way_sweep(C.byref(self.p), C.byref(self.opts))
+        if n < 0:
+            raise RuntimeError("oracle two-way sweep failed")
+        return int(n)
+
     def boundary_vacuum(self):
         lib().orc_boundary_vacuum(C.byref(self.p), C.byref(self.opts))
 
@@ -226,11 +235,12 @@ class Oracle:
     def compute_keff(self) -> float:
         return float(lib().orc_compute_keff(C.byref(self.p), C.byref(self.opts)))
 
-    def iterate(self, n_iters: int, keff: float = 1.0, vacuum: bool = False):
+    def iterate(self, n_iters: int, keff: float = 1.0, vacuum: bool = False,
+                two_way: bool = False):
         """The reference's per-iteration call order (src/main.c:57-92)."""
         out = []
         for _ in range(n_iters):
-            segs = self.transport_sweep()
+            segs = self.two_way_sweep() if two_way else self.transport_sweep()
             if vacuum:
                 self.boundary_vacuum()
             self.renormalize_flux()
@@ -330,10 +340,12 @@ def dump_problem(dump: dict):
     return dims, arrays
 
 
-def run_ref_serial(input_file, gen_seed, sweep_seed, iters, dump_path) -> dict:
-    """Run the reference harness (serial -O0 build) and read its dump."""
+def run_ref_serial(input_file, gen_seed, sweep_seed, iters, dump_path, two_way=False) -> dict:
+    """Run the reference harness (serial -O0 build) and read its dump; with
+    two_way the sweep is the reference's two_way_transport_sweep."""
     if not os.path.exists(REF_SERIAL):
         raise FileNotFoundError(REF_SERIAL)
     subprocess.run([REF_SERIAL, str(input_file), str(gen_seed), str(sweep_seed), str(iters),
-                    str(dump_path)], check=True, stdout=subprocess.DEVNULL)
+                    str(dump_path)] + (["two_way"] if two_way else []), check=True,
+                   stdout=subprocess.DEVNULL)
     return read_dump(dump_path)

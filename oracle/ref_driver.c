@@ -9,7 +9,12 @@
  * seeds, and writes the generated problem and the resulting state to a
  * binary dump that tests/ and tests/golden/ read.
  *
- * Usage: ref_driver <input file> <gen_seed> <sweep_seed> <iters> <dump>
+ * Usage: ref_driver <input file> <gen_seed> <sweep_seed> <iters> <dump> [two_way]
+ *
+ * With "two_way" each iteration's sweep is the reference's
+ * two_way_transport_sweep (src/solver.c:556-891, defined there but not
+ * declared in its header) instead of transport_sweep, and b_psi is dumped
+ * too.
  *
  * Dump format: a sequence of records
  *   u32 name_len, name bytes, u32 type (0 f32, 1 i64, 2 i32, 3 f64), u64 count, data
@@ -17,6 +22,8 @@
 #include "SimpleMOC_header.h"
 
 static FILE *g_out;
+
+void two_way_transport_sweep(Params *params, Input *I);
 
 static void put(const char *name, unsigned type, const void *data, unsigned long long count)
 {
@@ -31,10 +38,12 @@ static void put(const char *name, unsigned type, const void *data, unsigned long
 
 int main(int argc, char *argv[])
 {
-    if (argc != 6) {
-        fprintf(stderr, "usage: %s <input> <gen_seed> <sweep_seed> <iters> <dump>\n", argv[0]);
+    if (argc != 6 && !(argc == 7 && strcmp(argv[6], "two_way") == 0)) {
+        fprintf(stderr, "usage: %s <input> <gen_seed> <sweep_seed> <iters> <dump> [two_way]\n",
+                argv[0]);
         return 2;
     }
+    const int two_way = argc == 7;
     unsigned gen_seed = (unsigned)strtoul(argv[2], NULL, 10);
     unsigned sweep_seed = (unsigned)strtoul(argv[3], NULL, 10);
     int iters = atoi(argv[4]);
@@ -120,7 +129,10 @@ int main(int argc, char *argv[])
     float *keffs = malloc((iters + 1) * sizeof(float));
     float *res = malloc((iters + 1) * sizeof(float));
     for (int it = 0; it < iters; it++) {
-        transport_sweep(&params, &I);
+        if (two_way)
+            two_way_transport_sweep(&params, &I);
+        else
+            transport_sweep(&params, &I);
         segs[it] = I.segments_processed;
         if (it == 0) {
             put("flux_after_sweep0", 0, params.sources[0].fine_flux[0], nsr * fai * G);
@@ -145,6 +157,11 @@ int main(int argc, char *argv[])
     for (long t = 0; t < T; t++)
         memcpy(psi + t * G, td[t].f_psi, G * sizeof(float));
     put("f_psi", 0, psi, T * G);
+    if (two_way) {
+        for (long t = 0; t < T; t++)
+            memcpy(psi + t * G, td[t].b_psi, G * sizeof(float));
+        put("b_psi", 0, psi, T * G);
+    }
     float lk = *params.leakage;
     put("leakage", 0, &lk, 1);
     fclose(g_out);

@@ -139,6 +139,7 @@ def lib():
             "moc_create": ([P, O, C.POINTER(C.c_void_p)], C.c_int),
             "moc_destroy": ([C.c_void_p], C.c_int),
             "moc_transport_sweep": ([C.c_void_p, C.POINTER(C.c_long)], C.c_int),
+            "moc_two_way_sweep": ([C.c_void_p, C.POINTER(C.c_long)], C.c_int),
             "moc_boundary_exchange": ([C.c_void_p], C.c_int),
             "moc_renormalize_flux": ([C.c_void_p], C.c_int),
             "moc_update_sources": ([C.c_void_p, C.c_float, _fp], C.c_int),
@@ -409,6 +410,14 @@ class Solver:
         _check(lib().moc_transport_sweep(self._h, C.byref(n)))
         return n.value
 
+    # src/solver.c:556
+    def two_way_sweep(self) -> int:
+        """Forward trace into f_psi, backward replay into b_psi; the count
+        is 2 x pieces for the quadratic source, 0 for the flat one."""
+        n = C.c_long()
+        _check(lib().moc_two_way_sweep(self._h, C.byref(n)))
+        return n.value
+
     # src/comms.c:5
     def boundary_exchange(self):
         _check(lib().moc_boundary_exchange(self._h))
@@ -429,11 +438,12 @@ class Solver:
         _check(lib().moc_compute_keff(self._h, C.byref(k)))
         return k.value
 
-    def iterate(self, n_iters, keff=1.0):
-        """src/main.c:57-92 call order; returns [(segments, keff, residual)]."""
+    def iterate(self, n_iters, keff=1.0, two_way=False):
+        """src/main.c:57-92 call order; returns [(segments, keff, residual)].
+        two_way: the sweep is two_way_transport_sweep (src/solver.c:556)."""
         out = []
         for _ in range(n_iters):
-            segs = self.transport_sweep()
+            segs = self.two_way_sweep() if two_way else self.transport_sweep()
             self.boundary_exchange()
             self.renormalize_flux()
             res = self.update_sources(keff)
@@ -447,8 +457,8 @@ class Solver:
         _check(lib().moc_download(self._h, p._p, what))
         return p
 
-    def state(self, psi: bool = True):
-        """Fresh numpy copies of flux, source, (psi,) z and leakage."""
+    def state(self, psi: bool = True, b_psi: bool = False):
+        """Fresh numpy copies of flux, source, (psi, b_psi,) z and leakage."""
         I = self.problem._p.contents.inp
         T, G = I.ntracks, I.n_egroups
         nv = I.n_source_regions_per_node * I.fai * G
@@ -458,6 +468,9 @@ class Solver:
         if psi:
             arrs["f_psi"] = np.zeros(T * G, np.float32)
             what |= DL_PSI
+        if b_psi:
+            arrs["b_psi"] = np.zeros(T * G, np.float32)
+            what |= DL_B_PSI
         p = _Problem()
         p.inp = I
         for k, a in arrs.items():

@@ -100,6 +100,15 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a)
         trace_stack<WRITE ? TRACE_WRITE : TRACE_COUNT, KPT>(a, s, a.sweep_key, s_w);
 }
 
+// two-way sweep (src/solver.c:556-891): one workgroup per z-stack
+template <bool WRITE, int KPT>
+__global__ __launch_bounds__(256) void k_trace_tw(TraceArgs a)
+{
+    __shared__ int s_w[16];
+    for (long long s = a.s_begin + blockIdx.x; s < a.s_end; s += gridDim.x)
+        trace_stack_tw<WRITE ? TRACE_WRITE : TRACE_COUNT, KPT>(a, s, a.sweep_key, s_w);
+}
+
 // 256/LPT tracks per block
 template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
 __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
@@ -1108,7 +1117,7 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
     const long long nt2 = c->nt2, T = c->T, G = c->G, nsr = c->nsr;
     const long long nst = nt2 * c->np;
     if ((rc = dalloc(&c->d_segoff, nt2 + 1)) || (rc = dalloc(&c->d_seglen, c->n_seg2d)) ||
-        (rc = dalloc(&c->d_az, nt2)) || (rc = dalloc(&c->d_mu, c->np)) ||
+        (rc = dalloc(&c->d_az, nt2)) || (rc = dalloc(&c->d_mu, 2 * c->np)) ||
         (rc = dalloc(&c->d_cosd, c->np)) || (rc = dalloc(&c->d_sind, c->np)) ||
         (rc = dalloc(&c->d_pw, T)) || (rc = dalloc(&c->d_z, T)) ||
         (rc = dalloc(&c->d_psi, (size_t)T * G)) || (rc = dalloc(&c->d_src, c->nval)) ||
@@ -1131,12 +1140,14 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
     std::vector<long long> segoff(nt2 + 1);
     for (long long i = 0; i <= nt2; i++)
         segoff[i] = p->seg_offset[i];
-    std::vector<float> mu(c->np);
+    // mu, then -mu (the backward replay of two-way sweeps)
+    std::vector<float> mu(2 * (size_t)c->np);
     std::vector<double> cosd(c->np), sind(c->np);
     for (int j = 0; j < c->np; j++) {
         cosd[j] = cos((double)p->polar_angle[j]);
         sind[j] = sin((double)p->polar_angle[j]);
         mu[j] = (float)cosd[j];
+        mu[c->np + j] = -mu[j];
     }
     // scattering transposed per material: scT[m][g2][g] = scatter[m][g][g2]
     std::vector<float> scT((size_t)c->n_mat * G * G);
@@ -1157,7 +1168,7 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
     };
     if ((rc = up(c->d_segoff, segoff.data(), (nt2 + 1) * 8)) ||
         (rc = up(c->d_seglen, p->seg_length, c->n_seg2d * 4)) || (rc = up(c->d_az, p->az_weight, nt2 * 4)) ||
-        (rc = up(c->d_mu, mu.data(), c->np * 4)) || (rc = up(c->d_cosd, cosd.data(), c->np * 8)) ||
+        (rc = up(c->d_mu, mu.data(), c->np * 8)) || (rc = up(c->d_cosd, cosd.data(), c->np * 8)) ||
         (rc = up(c->d_sind, sind.data(), c->np * 8)) || (rc = up(c->d_pw, p->p_weight, T * 4)) ||
         (rc = up(c->d_z, p->z_height, T * 4)) || (rc = up(c->d_src, p->fine_source, c->nval * 4)) ||
         (rc = up(c->d_flux, p->fine_flux, c->nval * 4)) || (rc = up(c->d_sig, p->sigT, nsr * G * 4)) ||
@@ -1261,6 +1272,7 @@ static TraceArgs trace_args(moc_ctx *c)
     a.rec_base = 0;
     a.s_begin = 0;
     a.s_end = c->nt2 * c->np;
+    a.nint = c->cai * c->fai;
     return a;
 }
 
@@ -1323,19 +1335,23 @@ static void launch_attenuate(moc_ctx *c, const AttArgs &a)
 // one workgroup per z-stack; KPT stacked tracks per thread
 static int trace_kpt(int zs) { return zs <= 256 ? 1 : zs <= 512 ? 2 : zs <= 1024 ? 4 : 8; }
 
-static void launch_trace(moc_ctx *c, const TraceArgs &ta, bool write)
+static void launch_trace(moc_ctx *c, const TraceArgs &ta, bool write, bool two_way = false)
 {
     const long long nst = ta.s_end - ta.s_begin;
     const int kpt = trace_kpt(c->zs);
     int nt = (c->zs + kpt - 1) / kpt;
     nt = ((nt + 63) / 64) * 64;
     const int grid = (int)(nst < (1 << 20) ? (nst < 1 ? 1 : nst) : (1 << 20));
-#define MOC_TRACE_LAUNCH(K)                                                                     \
-    do {                                                                                       \
-        if (write)                                                                             \
-            hipLaunchKernelGGL((k_trace<true, K>), dim3(grid), dim3(nt), 0, c->stream, ta);    \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_trace<false, K>), dim3(grid), dim3(nt), 0, c->stream, ta);   \
+#define MOC_TRACE_LAUNCH(K)                                                                      \
+    do {                                                                                        \
+        if (two_way && write)                                                                   \
+            hipLaunchKernelGGL((k_trace_tw<true, K>), dim3(grid), dim3(nt), 0, c->stream, ta);  \
+        else if (two_way)                                                                       \
+            hipLaunchKernelGGL((k_trace_tw<false, K>), dim3(grid), dim3(nt), 0, c->stream, ta); \
+        else if (write)                                                                         \
+            hipLaunchKernelGGL((k_trace<true, K>), dim3(grid), dim3(nt), 0, c->stream, ta);     \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_trace<false, K>), dim3(grid), dim3(nt), 0, c->stream, ta);    \
     } while (0)
     switch (kpt) {
     case 1: MOC_TRACE_LAUNCH(1); break;
@@ -1419,15 +1435,17 @@ static int records_budget(moc_ctx *c, long long S, long long reserve, long long 
 
 // Count pass and offsets of the next sweep (z unchanged), then the chunks:
 // one when all records fit, else runs of whole stacks of at most the
-// budget's records each.
+// budget's records each.  A two-way sweep keeps two records per piece
+// (forward, then backward at chunk offset S).
 constexpr long long RECORD_RESERVE = 4LL << 30;
 
 static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
-                      long long reserve = RECORD_RESERVE)
+                      long long reserve = RECORD_RESERVE, bool two_way = false)
 {
     const long long nst = c->nt2 * c->np;
+    const long long per = two_way ? 2 : 1;
     TraceArgs ta = trace_args(c);
-    launch_trace(c, ta, false);
+    launch_trace(c, ta, false, two_way);
     HIP_TRY(hipGetLastError());
     size_t tb = c->scan_tmp_bytes;
     HIP_TRY(rocprim::exclusive_scan(c->d_scan_tmp, tb, c->d_cnt, c->d_off, (long long)0,
@@ -1436,18 +1454,19 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
     HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     long long budget = 0;
-    TRY(records_budget(c, S, reserve, &budget));
+    TRY(records_budget(c, per * S, reserve, &budget));
+    const long long pieces = budget / per;   // pieces one chunk may hold
     chunks.clear();
-    if (S <= budget) {
+    if (S <= pieces) {
         chunks.push_back({0, nst, 0, S});
-        TRY(ensure_records(c, S, budget));
+        TRY(ensure_records(c, per * S, budget));
     } else {
         std::vector<long long> sc(nst);
         HIP_TRY(hipMemcpy(sc.data(), c->d_stack_cnt, nst * 8, hipMemcpyDeviceToHost));
         long long base = 0, biggest = 0;
         long long s0 = 0, acc = 0;
         for (long long s = 0; s < nst; s++) {
-            if (acc > 0 && acc + sc[s] > budget) {
+            if (acc > 0 && acc + sc[s] > pieces) {
                 chunks.push_back({s0, s, base, acc});
                 biggest = std::max(biggest, acc);
                 base += acc;
@@ -1458,17 +1477,17 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
         }
         chunks.push_back({s0, nst, base, acc});
         biggest = std::max(biggest, acc);
-        if (biggest > budget && c->opt.max_records == 0)
+        if (biggest > pieces && c->opt.max_records == 0)
             return moc_set_error(MOC_ERR_NOMEM, "one z-stack needs %lld piece records, more than "
-                                 "device memory holds", biggest);
+                                 "device memory holds", per * biggest);
         // exact fit: every chunk reuses the one buffer
-        if (biggest > c->rec_cap || c->rec_cap > budget) {
+        if (per * biggest > c->rec_cap || c->rec_cap > budget) {
             if (c->d_rec)
                 HIP_TRY(hipFree(c->d_rec));
             c->d_rec = nullptr;
             c->rec_cap = 0;
-            TRY(dalloc(&c->d_rec, biggest));
-            c->rec_cap = biggest;
+            TRY(dalloc(&c->d_rec, per * biggest));
+            c->rec_cap = per * biggest;
         }
     }
     if (c->opt.qsr_mode == MOC_QSR_LIBC) {
@@ -1485,12 +1504,18 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
 // rand() draws, the chunk's draws come next in the serial order
 // (src/solver.c:481: stack-major, then the trace order), which is the
 // chunk's record range.
-static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit, const uint32_t *wmask = nullptr)
+static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit, const uint32_t *wmask = nullptr,
+                       bool two_way = false)
 {
     TraceArgs ta = trace_args(c);
     ta.commit = commit ? 1 : 0;
     ta.rec = c->d_rec;
     ta.rec_cap = c->rec_cap;
+    if (two_way) {
+        // forward records in [0, S), backward ones in [S, 2S)
+        ta.rec_cap = ch.S;
+        ta.rec_b = c->d_rec + ch.S;
+    }
     ta.rec_base = ch.rec_base;
     ta.s_begin = ch.s0;
     ta.s_end = ch.s1;
@@ -1518,12 +1543,12 @@ static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit, const uint32_t 
         // the previous chunk's write pass has read the buffer before this copy
         HIP_TRY(hipMemcpyAsync(c->d_stream, draws.data(), ch.S * 4, hipMemcpyHostToDevice, c->stream));
         ta.qsr_stream = c->d_stream;
-        launch_trace(c, ta, true);
+        launch_trace(c, ta, true, two_way);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(c->stream));   // draws stays alive until here
         return MOC_OK;
     }
-    launch_trace(c, ta, true);
+    launch_trace(c, ta, true, two_way);
     HIP_TRY(hipGetLastError());
     return MOC_OK;
 }
@@ -1579,11 +1604,38 @@ static AttArgs att_args(moc_ctx *c, const Chunk &ch)
     return aa;
 }
 
-extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+// b_psi on the device from the first two-way sweep on: until then it was
+// only ever scaled from zero, so it is 0, or NaN once a normalisation
+// factor was not finite
+__global__ void k_init_bpsi(float *__restrict__ x, long long n, const float *__restrict__ scal)
 {
-    if (!c)
-        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
+    const float v = scal[SC_BPSI_NAN] != 0.0f ? NAN : 0.0f;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x)
+        x[e] = v;
+}
+
+static int ensure_bpsi(moc_ctx *c)
+{
+    if (c->d_bpsi)
+        return MOC_OK;
+    const long long n = c->T * c->G;
+    TRY(dalloc(&c->d_bpsi, (size_t)n));
+    hipLaunchKernelGGL(k_init_bpsi, dim3(launch_grid(n, 256)), dim3(256), 0, c->stream, c->d_bpsi, n,
+                       c->d_scal);
+    HIP_TRY(hipGetLastError());
+    return MOC_OK;
+}
+
+// One sweep: the reference's transport_sweep (src/solver.c:283-552) or,
+// with two_way, two_way_transport_sweep (src/solver.c:556-891), whose
+// records are attenuated twice per chunk: forward into psi with mu, the
+// backward replay into b_psi with -mu.
+static int sweep(moc_ctx *c, bool two_way, long *segments_processed)
+{
     HIP_TRY(hipSetDevice(c->device));
+    if (two_way)
+        TRY(ensure_bpsi(c));
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     // (1) coefficients + accumulator
     TRY(prep_coefficients(c, c->d_acc));
@@ -1592,7 +1644,7 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     // (z heights committed) and their attenuation
     long long S = 0;
     std::vector<Chunk> chunks;
-    TRY(plan_trace(c, chunks, &S));
+    TRY(plan_trace(c, chunks, &S, RECORD_RESERVE, two_way));
     float ms_trace = 0.f, ms_att = 0.f;
     {
         float ms = 0.f;
@@ -1603,11 +1655,21 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     }
     for (size_t k = 0; k < chunks.size(); k++) {
         HIP_TRY(hipEventRecord(c->ev[5], c->stream));
-        TRY(write_chunk(c, chunks[k], true));
+        TRY(write_chunk(c, chunks[k], true, nullptr, two_way));
         note_chunk(c, chunks[k]);
         HIP_TRY(hipEventRecord(c->ev[6], c->stream));
-        launch_attenuate(c, att_args(c, chunks[k]));
+        const AttArgs fwd = att_args(c, chunks[k]);
+        launch_attenuate(c, fwd);
         HIP_TRY(hipGetLastError());
+        if (two_way) {
+            AttArgs bwd = fwd;
+            bwd.rec = c->d_rec + chunks[k].S;
+            bwd.psi = c->d_bpsi;
+            bwd.psi_scale = nullptr;   // renormalisation scales b_psi in place
+            bwd.mu = c->d_mu + c->np;
+            launch_attenuate(c, bwd);
+            HIP_TRY(hipGetLastError());
+        }
         HIP_TRY(hipEventRecord(c->ev[7], c->stream));
         HIP_TRY(hipEventSynchronize(c->ev[7]));
         float a = 0.f, b = 0.f;
@@ -1640,13 +1702,30 @@ extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
     c->timing.finalize_ms = ms_fin;
     c->timing.sweep_ms = ms_all;
     c->timing.chunks = c->chunks;
-    c->last_segments = (long)S;
-    c->timing.segments = (long)S;
+    // the two-way sweep counts quadratic-source pieces only, both ways
+    // (src/solver.c:782, :827)
+    const long long segs = two_way ? (c->quad ? 2 * S : 0) : S;
+    c->last_segments = (long)segs;
+    c->timing.segments = (long)segs;
     c->sweeps++;
     c->timing.sweeps = c->sweeps;
     if (segments_processed)
-        *segments_processed = (long)S;
+        *segments_processed = (long)segs;
     return MOC_OK;
+}
+
+extern "C" int moc_transport_sweep(moc_ctx *c, long *segments_processed)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_transport_sweep: null context");
+    return sweep(c, false, segments_processed);
+}
+
+extern "C" int moc_two_way_sweep(moc_ctx *c, long *segments_processed)
+{
+    if (!c)
+        return moc_set_error(MOC_ERR_INVALID, "moc_two_way_sweep: null context");
+    return sweep(c, true, segments_processed);
 }
 
 // src/comms.c:5-196 for one domain: the 6 faces have no neighbour, so each

@@ -1,7 +1,8 @@
 // moc_dropin.cpp -- the reference's four solver symbols on top of the C ABI.
 //
 // transport_sweep / renormalize_flux / update_sources / compute_keff keep
-// the signatures of src/SimpleMOC_header.h:225-237.  On the first call the
+// the signatures of src/SimpleMOC_header.h:225-237, and
+// two_way_transport_sweep that of its definition at src/solver.c:556.  On the first call the
 // reference's pointer structures are flattened into a moc_problem and a
 // device context is created; later calls reuse it while the Params
 // pointers and shape stay the same.  After each call the state the
@@ -308,6 +309,30 @@ extern "C" void transport_sweep(moc_ref_Params *params, moc_ref_Input *I)
     if (moc_transport_sweep(g_state.ctx, &segs) != MOC_OK)
         die("transport sweep");
     write_back(params, I, MOC_DL_FLUX | MOC_DL_PSI | MOC_DL_Z);
+    I->segments_processed = segs;
+}
+
+// src/solver.c:556-891 (defined there, not declared in the reference's
+// header).  The caller's b_psi arrays are mirrored from the first call on.
+extern "C" void two_way_transport_sweep(moc_ref_Params *params, moc_ref_Input *I)
+{
+    if (I->mype == 0)
+        printf("Starting transport sweep ...\n");
+    if (I->axial_exp != 0 && I->axial_exp != 2) {
+        printf("Error: invalid axial expansion order");
+        printf("\n Please input 0 or 2\n");
+        exit(1);
+    }
+    ensure(params, I);
+    DropIn &s = g_state;
+    if (s.bpsi.empty()) {
+        s.bpsi.assign((size_t)s.ntracks * s.G, 0.0f);
+        s.prob.b_psi = s.bpsi.data();
+    }
+    long segs = 0;
+    if (moc_two_way_sweep(s.ctx, &segs) != MOC_OK)
+        die("two-way transport sweep");
+    write_back(params, I, MOC_DL_FLUX | MOC_DL_PSI | MOC_DL_B_PSI | MOC_DL_Z);
     I->segments_processed = segs;
 }
 

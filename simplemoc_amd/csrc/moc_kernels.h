@@ -64,6 +64,11 @@ struct TraceArgs {
     int *sort_key;
     int *sort_id;
     long long key_base;
+    // two-way sweeps: fine intervals per node, and where the write pass
+    // puts the backward records (track t's at the same offsets as its
+    // forward ones)
+    int nint;
+    float4 *rec_b;
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.

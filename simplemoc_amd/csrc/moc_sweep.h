@@ -136,6 +136,53 @@ struct EmitCtx {
     const uint32_t *wmask;      // FSRs to watch for SegResult::last_weighted (or null)
 };
 
+// {ds, zin, fsr} of a piece of region qsr starting at height zk, at record
+// index q (zin and the fine interval as src/solver.c:38-45, with the
+// quadratic fit's edge shift); returns the FSR.
+__device__ __forceinline__ uint32_t store_record(const EmitCtx *ec, long long q, float zk, float ds,
+                                                 uint32_t qsr)
+{
+    const int cell = (int)(zk / ec->dz_att);
+    float zin = zk - ec->dz_att * ((float)cell + 0.5f);
+    int fine = cell % ec->fai;
+    if (ec->quad) {
+        if (fine == 0)
+            zin -= ec->dz_att;
+        else if (fine == ec->fai - 1)
+            zin += ec->dz_att;
+    }
+    if (fine < 0) {
+        atomicOr(ec->err, ERR_FINE_RANGE);
+        fine = 0;
+    }
+    const uint32_t fsr = qsr * (uint32_t)ec->fai + (uint32_t)fine;
+    if (q < ec->rec_cap)
+        ec->rec[q] = make_float4(ds, zin, __uint_as_float(fsr), 0.f);
+    else
+        atomicOr(ec->err, ERR_RECORD_CAP);
+    return fsr;
+}
+
+// Record of piece p of segment n of the track in ec, which starts at
+// height zk: its region (the track's hash or the next explicit draw), the
+// cell of its start as attenuate_fluxes computes it (src/solver.c:38-45),
+// stored at ec->base + p.  Returns the piece's FSR.
+__device__ __forceinline__ uint32_t emit_piece(const EmitCtx *ec, long long n, int p, float zk, float ds)
+{
+    uint32_t qsr;
+    if (ec->stream) {
+        int32_t v = ec->stream[ec->serial + p];
+        if (v < 0 || (long long)v >= ec->nsr) {
+            atomicOr(ec->err, ERR_QSR_RANGE);
+            v = 0;
+        }
+        qsr = (uint32_t)v;
+    } else {
+        qsr = piece_region(ec->tkey, n, p, ec->nsr);
+    }
+    return store_record(ec, ec->base + p, zk, ds, qsr);
+}
+
 // One (track, 2D segment): the while loop of src/solver.c:409-525.
 template <bool EMIT>
 __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k, bool pos,
@@ -170,37 +217,7 @@ __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k
             }
         }
         if (EMIT) {
-            uint32_t qsr;
-            if (ec->stream) {
-                int32_t v = ec->stream[ec->serial + p];
-                if (v < 0 || (long long)v >= ec->nsr) {
-                    atomicOr(ec->err, ERR_QSR_RANGE);
-                    v = 0;
-                }
-                qsr = (uint32_t)v;
-            } else {
-                qsr = piece_region(ec->tkey, n, p, ec->nsr);
-            }
-            // cell of the piece start, as attenuate_fluxes computes it (:38-45)
-            const int cell = (int)(zk / ec->dz_att);
-            float zin = zk - ec->dz_att * ((float)cell + 0.5f);
-            int fine = cell % ec->fai;
-            if (ec->quad) {
-                if (fine == 0)
-                    zin -= ec->dz_att;
-                else if (fine == ec->fai - 1)
-                    zin += ec->dz_att;
-            }
-            if (fine < 0) {
-                atomicOr(ec->err, ERR_FINE_RANGE);
-                fine = 0;
-            }
-            const long long q = ec->base + p;
-            const uint32_t fsr = qsr * (uint32_t)ec->fai + (uint32_t)fine;
-            if (q < ec->rec_cap)
-                ec->rec[q] = make_float4(ds, zin, __uint_as_float(fsr), 0.f);
-            else
-                atomicOr(ec->err, ERR_RECORD_CAP);
+            const uint32_t fsr = emit_piece(ec, n, p, zk, ds);
             if (ec->wmask && ((ec->wmask[fsr >> 5] >> (fsr & 31)) & 1u))
                 last_w = p;
         }
@@ -456,6 +473,203 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------
+// (2b) two-way axial trace (src/solver.c:556-891).  Per 2D segment every
+// stacked track in [begin, end) traces its pieces with the reference's
+// own interval arithmetic: the starting interval is (int)z / fine_dz (the
+// cast binds to z alone, :671), the segment is cut at the distance to the
+// node boundary, and the tracks that reach it only move the bounds after
+// the whole k loop (end -= completed going up, begin += completed going
+// down, :805-808), so no prefix test is needed.  z is never reset on the
+// way; the final heights start the backward replay.
+struct TwSeg {
+    int pieces;
+    int completed;
+    float zfin;
+};
+
+template <bool EMIT>
+__device__ __forceinline__ TwSeg trace_segment_tw(float z, float s_full, bool pos, float mu, long long n,
+                                                  const TraceArgs &a, const EmitCtx *ec)
+{
+    int interval = (int)((double)(int)z / a.fine_dz);
+    const float bound = pos ? (float)((a.node_dz - (double)z) / (double)mu) : -z / mu;
+    float s;
+    int completed = 0;
+    if (s_full < bound) {
+        s = s_full;
+    } else {
+        s = bound;
+        completed = 1;
+    }
+    bool done = false;
+    int p = 0;
+    while (!done) {
+        const float fz = pos ? (float)((double)(interval + 1) * a.fine_dz)
+                             : (float)((double)interval * a.fine_dz);
+        const float zd = fz - z;
+        const float sd = zd / mu;
+        float ds, zn;
+        if (sd < s) {
+            interval += pos ? 1 : -1;
+            ds = sd;
+            zn = z + zd;
+        } else {
+            ds = s;
+            zn = z + s * mu;
+        }
+        s -= ds;
+        if (s <= 0.f || interval < 0 || interval >= a.nint)
+            done = true;
+        if (EMIT)
+            (void)emit_piece(ec, n, p, z, ds);
+        p++;
+        if (p >= MAX_PIECES_PER_SEGMENT && !done) {
+            atomicOr(a.err, ERR_TRACE_RUNAWAY);
+            done = true;
+        }
+        z = zn;
+    }
+    TwSeg r;
+    r.pieces = p;
+    r.completed = completed;
+    r.zfin = z;
+    return r;
+}
+
+// One z-stack of a two-way sweep (TRACE_COUNT or TRACE_WRITE).  The write
+// pass stores each track's forward records at off[t], then replays them
+// backwards into rec_b at the same offsets: piece q of the replay is
+// forward piece cnt-1-q with the region kept and the cell taken from the
+// replay height, which starts at the final forward height and drops by
+// ds*mu after every piece (:811-842).  With commit, z returns to the
+// track's starting height (:846-853).
+template <int MODE, int KPT>
+__device__ __forceinline__ void trace_stack_tw(const TraceArgs &a, long long s, uint64_t skey, int *s_w)
+{
+    static_assert(MODE == TRACE_COUNT || MODE == TRACE_WRITE, "two-way trace modes");
+    constexpr bool WRITE = MODE == TRACE_WRITE;
+    const int zs = a.zs;
+    const int k0 = threadIdx.x * KPT;
+    const long long i = s / a.np;
+    const int j = (int)(s - i * a.np);
+    const long long t0 = s * zs;
+    const bool pos = j < a.np / 2;
+    const float mu = (float)a.cosd[j];
+    const double sn = a.sind[j];
+    float z[KPT];
+    int cnt[KPT];
+    long long base[KPT];
+    uint64_t tkey[KPT];
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = k0 + r;
+        z[r] = (k < zs) ? a.z[t0 + k] : 0.f;
+        cnt[r] = 0;
+        base[r] = (WRITE && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
+        tkey[r] = (WRITE && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
+    }
+    EmitCtx ec;
+    ec.rec = a.rec;
+    ec.rec_cap = a.rec_cap;
+    ec.stream = a.qsr_stream;
+    ec.serial = 0;
+    ec.nsr = a.nsr;
+    ec.fai = a.fai;
+    ec.quad = a.quad;
+    ec.dz_att = a.dz_att;
+    ec.err = a.err;
+    ec.wmask = nullptr;
+    // hash draws are emitted in the first trace; explicit draws need the
+    // serial position of each track's first piece, known after a scan
+    const bool direct = WRITE && !a.qsr_stream;
+    int begin = 0, end = zs;
+    long long serial = (WRITE && a.qsr_stream) ? a.stack_off[s] - a.rec_base : 0;
+    const long long off = a.seg_off[i];
+    const long long nseg = a.seg_off[i + 1] - off;
+    for (long long n = 0; n < nseg; n++) {
+        const float s_full = (float)((double)a.seg_len[off + n] / sn);
+        TwSeg res[KPT];
+        bool act[KPT];
+        int ncomp = 0, npieces = 0;
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = k0 + r;
+            act[r] = (k >= begin && k < end);
+            if (act[r] && direct) {
+                ec.base = base[r] + cnt[r];
+                ec.tkey = tkey[r];
+                res[r] = trace_segment_tw<true>(z[r], s_full, pos, mu, n, a, &ec);
+            } else if (act[r]) {
+                res[r] = trace_segment_tw<false>(z[r], s_full, pos, mu, n, a, nullptr);
+            } else {
+                res[r].pieces = 0;
+                res[r].completed = 0;
+                res[r].zfin = z[r];
+            }
+            ncomp += res[r].completed;
+            npieces += res[r].pieces;
+        }
+        if (WRITE && a.qsr_stream) {
+            int ptot;
+            long long ser = serial + block_excl_scan(npieces, s_w, &ptot);
+            serial += ptot;
+#pragma unroll
+            for (int r = 0; r < KPT; r++) {
+                if (!act[r])
+                    continue;
+                ec.base = base[r] + cnt[r];
+                ec.serial = ser;
+                (void)trace_segment_tw<true>(z[r], s_full, pos, mu, n, a, &ec);
+                ser += res[r].pieces;
+            }
+        }
+        int ctot;
+        (void)block_excl_scan(ncomp, s_w, &ctot);
+        if (pos)
+            end -= ctot;
+        else
+            begin += ctot;
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            cnt[r] += res[r].pieces;
+            z[r] = res[r].zfin;
+        }
+    }
+    long long stack_total = 0;
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = k0 + r;
+        if (k >= zs)
+            continue;
+        stack_total += cnt[r];
+        if (!WRITE) {
+            a.cnt[t0 + k] = cnt[r];
+            continue;
+        }
+        // backward replay records
+        EmitCtx eb = ec;
+        eb.rec = a.rec_b;
+        float zb = z[r];
+        for (int q = cnt[r] - 1; q >= 0; q--) {
+            const long long qf = base[r] + q;
+            const float4 f = qf < a.rec_cap ? a.rec[qf] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t qsr = __float_as_uint(f.z) / (uint32_t)a.fai;
+            (void)store_record(&eb, base[r] + (cnt[r] - 1 - q), zb, f.x, qsr);
+            zb -= f.x * mu;
+        }
+        if (a.commit)
+            a.z[t0 + k] = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
+    }
+    if (MODE == TRACE_COUNT && a.stack_cnt) {
+        int tot;
+        (void)block_excl_scan((int)stack_total, s_w, &tot);
+        if (threadIdx.x == 0)
+            a.stack_cnt[s] = tot;
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------

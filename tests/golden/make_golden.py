@@ -7,7 +7,14 @@ sequence (src/main.c:57-92) with fixed seeds.  Stored per case:
     MOC_GEN_REFERENCE generator must reproduce them);
   - per-iteration segments_processed, keff and source residual;
   - fine_flux and z heights after the first sweep;
-  - final fine_flux, fine_source, z heights, forward psi and leakage.
+  - final fine_flux, fine_source, z heights, forward psi and leakage;
+  - for the two-way cases (each sweep is the reference's
+    two_way_transport_sweep, src/solver.c:556-891), backward psi too.
+    Their inputs have a whole-number fine interval height (height /
+    decomp / (cai * fai) cm): with a fractional one the reference's
+    upward interval (int)z / fine_delta_z (src/solver.c:671) can lag by
+    one, giving negative piece lengths and exponential-table reads before
+    the table's start (src/solver.c:1447), i.e. undefined values.
 
 Usage: python tests/golden/make_golden.py   (needs the reference tree to
 build oracle/_ref; the fixtures themselves carry no reference code)
@@ -29,11 +36,13 @@ sys.path.insert(0, ROOT)
 
 from oracle import pyoracle as po  # noqa: E402
 
-# (name, input file, gen_seed, sweep_seed, iterations)
+# (name, input file, gen_seed, sweep_seed, iterations, two-way sweep)
 CASES = [
-    ("tiny_quad", "tiny.in", 7, 11, 3),
-    ("tiny_flat", "tiny_flat.in", 5, 13, 3),
-    ("tiny_g16_decomp2", "tiny_g16.in", 9, 17, 2),
+    ("tiny_quad", "tiny.in", 7, 11, 3, False),
+    ("tiny_flat", "tiny_flat.in", 5, 13, 3, False),
+    ("tiny_g16_decomp2", "tiny_g16.in", 9, 17, 2, False),
+    ("two_way_quad", "tiny_tw.in", 21, 23, 2, True),
+    ("two_way_flat", "tiny_tw_flat.in", 25, 27, 2, True),
 ]
 
 PROBLEM_KEYS = ["az_weight", "seg_offset", "seg_length", "polar_angle", "p_weight",
@@ -48,17 +57,19 @@ def main():
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True,
                    stdout=subprocess.DEVNULL)
     index = {}
-    for name, inp, gs, ss, iters in CASES:
+    for name, inp, gs, ss, iters, two_way in CASES:
         with tempfile.TemporaryDirectory() as td:
             d = po.run_ref_serial(os.path.join(HERE, inp), gs, ss, iters,
-                                  os.path.join(td, "dump.bin"))
+                                  os.path.join(td, "dump.bin"), two_way=two_way)
+        extra = dict(b_psi=d["b_psi"]) if two_way else {}
         np.savez_compressed(
             os.path.join(HERE, f"{name}.npz"),
             segments=d["segments"], keff=d["keff"], residual=d["residual"],
             flux_after_sweep0=d["flux_after_sweep0"], z_after_sweep0=d["z_after_sweep0"],
             fine_flux=d["fine_flux"], fine_source=d["fine_source"], z_height=d["z_height"],
-            f_psi=d["f_psi"], leakage=d["leakage"])
+            f_psi=d["f_psi"], leakage=d["leakage"], **extra)
         index[name] = dict(input=inp, gen_seed=gs, sweep_seed=ss, iterations=iters,
+                           two_way=two_way,
                            problem_sha256={k: digest(d[k]) for k in PROBLEM_KEYS},
                            dims=[int(x) for x in d["dims"]])
         print(name, "segments", d["segments"].tolist(), "keff", d["keff"].tolist())

@@ -1,0 +1,176 @@
+"""The two-way sweep (src/solver.c:556-891) on the GPU.
+
+Each stacked track is traced forward segment by segment (f_psi, mu), then
+its pieces are replayed backwards (b_psi, -mu), and z returns to its
+starting height.  Tolerances as for the one-way sweep:
+  * trace (segment counts, z heights): exact.
+  * GPU vs oracle in the same modes (hash region draws, FP64 tally, FP64
+    reductions): flux, source, f_psi and b_psi within NEAR = 1e-6
+    normwise, k_eff within 1e-6.
+  * GPU vs the reference's own two-way run (golden fixtures: libc rand()
+    draws, FP32 sequential tally, FP32 pairwise reductions): flux, psi
+    within REF_TOL = 1e-5 normwise, k_eff within 1e-5.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import simplemoc_amd as sm
+from helpers import GOLDEN, ROOT, flux_errors, make_input, oracle_for
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+NEAR = 1e-6
+REF_TOL = 1e-5
+DROPIN_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver_dropin")
+
+
+def assert_close(out_g, st, out_o, o, G, tol=NEAR, ktol=1e-6):
+    assert [x[0] for x in out_g] == [x[0] for x in out_o], "segment counts"
+    np.testing.assert_array_equal(st["z_height"], o["z_height"])
+    for (_, kg, _), (_, ko, _) in zip(out_g, out_o):
+        assert abs(kg - ko) <= ktol * max(1.0, abs(ko)), (kg, ko)
+    for k in ("fine_flux", "fine_source", "f_psi", "b_psi"):
+        norm, _ = flux_errors(st[k], o[k], G=G if k in ("fine_flux", "fine_source") else None)
+        assert norm <= tol, (k, norm)
+
+
+def run_both(prob, iters, opts=None, oracle_kw=None):
+    opts = opts or sm.Options()
+    o = oracle_for(prob, seed=opts.seed, **(oracle_kw or {}))
+    with sm.Solver(prob, opts) as s:
+        out_g = s.iterate(iters, 1.0, two_way=True)
+        st = s.state(b_psi=True)
+        tm = s.timing()
+    out_o = o.iterate(iters, 1.0, two_way=True)
+    return out_g, st, out_o, o, tm
+
+
+# height / (cai * fai): 0.5 and 0.6 are fractional (the reference's upward
+# interval lags and gives negative pieces, whose table cell both sides clamp
+# to the first); the others are whole numbers
+@pytest.mark.parametrize("G,fai,axial_exp,npol,height", [
+    (8, 5, 2, 4, 5.0), (16, 3, 2, 6, 9.0), (64, 4, 2, 4, 8.0), (128, 5, 2, 2, 10.0),
+    (100, 5, 2, 4, 6.0), (8, 3, 0, 4, 6.0), (32, 5, 0, 2, 5.0),
+])
+def test_two_way_hash_matches_oracle(G, fai, axial_exp, npol, height):
+    print(f"two-way hash G={G} fai={fai} axial_exp={axial_exp} npol={npol} height={height}",
+          flush=True)
+    inp = make_input(n_egroups=G, fai=fai, axial_exp=axial_exp, n_polar_angles=npol, height=height)
+    prob = sm.Problem.generate(inp, seed=G + 7 * fai, alloc_b_psi=True)
+    out_g, st, out_o, o, _ = run_both(prob, 3)
+    assert_close(out_g, st, out_o, o, G)
+    if axial_exp == 0:
+        assert all(x[0] == 0 for x in out_g)
+    # z ends every sweep at the starting heights
+    np.testing.assert_array_equal(st["z_height"], prob.array("z_height"))
+
+
+def test_two_way_fp32_tally_matches_oracle():
+    inp = make_input(n_egroups=32, height=10.0)
+    prob = sm.Problem.generate(inp, seed=4, alloc_b_psi=True)
+    out_g, st, out_o, o, _ = run_both(prob, 2, sm.Options(tally="f32"),
+                                      dict(tally=po.TALLY_F32, reduce=po.REDUCE_F64))
+    assert_close(out_g, st, out_o, o, 32, tol=REF_TOL, ktol=REF_TOL)
+
+
+def test_two_way_in_chunks_matches_oracle():
+    """Records for a few stacks at a time (forward and backward halves of
+    each chunk's buffer)."""
+    inp = make_input(n_egroups=16, height=10.0)
+    prob = sm.Problem.generate(inp, seed=6, alloc_b_psi=True)
+    out_g, st, out_o, o, tm = run_both(prob, 2, sm.Options(max_records=40000))
+    assert tm["chunks"] > 4, tm["chunks"]
+    assert_close(out_g, st, out_o, o, 16)
+
+
+def test_one_way_and_two_way_interleaved_with_vacuum():
+    """One-way, two-way, one-way sweeps with the vacuum boundary between:
+    b_psi is created zero at the first two-way sweep, tallied and cleared
+    by the boundary, scaled by every renormalisation."""
+    inp = make_input(n_egroups=8, height=20.0, radial_ray_sep=0.25)
+    prob = sm.Problem.generate(inp, seed=8, alloc_b_psi=True)
+    o = oracle_for(prob, seed=1)
+    seq = [False, True, True, False, True]
+    out_g, out_o = [], []
+    keff_g = keff_o = 1.0
+    with sm.Solver(prob, sm.Options(boundary="vacuum")) as s:
+        for tw in seq:
+            segs = s.two_way_sweep() if tw else s.transport_sweep()
+            s.boundary_exchange()
+            s.renormalize_flux()
+            res = s.update_sources(keff_g)
+            keff_g = s.compute_keff()
+            out_g.append((segs, keff_g, res))
+        st = s.state(b_psi=True)
+    for tw in seq:
+        segs = o.two_way_sweep() if tw else o.transport_sweep()
+        o.boundary_vacuum()
+        o.renormalize_flux()
+        res = o.update_sources(keff_o)
+        keff_o = o.compute_keff()
+        out_o.append((segs, keff_o, res))
+    assert_close(out_g, st, out_o, o, 8)
+    assert abs(st["leakage"] - o.leakage) <= NEAR * max(1.0, abs(o.leakage))
+
+
+@pytest.mark.parametrize("case", ["two_way_quad", "two_way_flat"])
+def test_two_way_matches_reference_golden_run(case, golden_index, libc):
+    """Reference-generated problem, libc rand() draws in the reference's
+    serial order: GPU vs the oracle's FP64 build to NEAR, GPU vs the
+    reference's own FP32 two-way run to REF_TOL."""
+    meta = golden_index[case]
+    g = np.load(os.path.join(GOLDEN, f"{case}.npz"))
+    inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
+    inp.derive()
+    prob = sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference", alloc_b_psi=True)
+    G = inp.n_egroups
+    iters = meta["iterations"]
+    with sm.Solver(prob, sm.Options(qsr="libc")) as s:
+        libc.srand(meta["sweep_seed"])
+        out = s.iterate(iters, two_way=True)
+        st = s.state(b_psi=True)
+    o64 = oracle_for(prob, qsr_mode=po.QSR_LIBC, tally=po.TALLY_F64, reduce=po.REDUCE_F64)
+    libc.srand(meta["sweep_seed"])
+    out64 = o64.iterate(iters, two_way=True)
+    assert [x[0] for x in out] == g["segments"].tolist()
+    np.testing.assert_array_equal(st["z_height"], g["z_height"])
+    for (_, k, _), kr, (_, k64, _) in zip(out, g["keff"], out64):
+        assert abs(k - kr) <= REF_TOL, (k, kr)
+        assert abs(k - k64) <= NEAR * max(1.0, abs(k64)), (k, k64)
+    for key in ("fine_flux", "f_psi", "b_psi"):
+        n64, _ = flux_errors(st[key], o64[key], G=G if key == "fine_flux" else None)
+        nref, _ = flux_errors(st[key], g[key], G=G if key == "fine_flux" else None)
+        print(f"{case} {key}: vs FP64 build {n64:.2e}, vs FP32 reference {nref:.2e}")
+        assert n64 <= NEAR, (key, n64)
+        assert nref <= REF_TOL, (key, nref)
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN_DRIVER), reason="drop-in driver harness not built")
+@pytest.mark.parametrize("case", ["two_way_quad", "two_way_flat"])
+def test_dropin_two_way_transport_sweep(case, golden_index, tmp_path):
+    """oracle/_ref/ref_driver_dropin: the golden-run driver over the
+    reference's generators with the library's two_way_transport_sweep,
+    renormalize_flux, update_sources and compute_keff in place of
+    solver.o, against the reference's own run of the same case."""
+    meta = golden_index[case]
+    g = np.load(os.path.join(GOLDEN, f"{case}.npz"))
+    dump = tmp_path / "dump.bin"
+    env = dict(os.environ, SIMPLEMOC_QSR="libc", SIMPLEMOC_DROPIN_OPENMP="0")
+    r = subprocess.run([DROPIN_DRIVER, os.path.join(GOLDEN, meta["input"]), str(meta["gen_seed"]),
+                        str(meta["sweep_seed"]), str(meta["iterations"]), str(dump), "two_way"],
+                       capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = po.read_dump(dump)
+    assert d["segments"].tolist() == g["segments"].tolist()
+    np.testing.assert_array_equal(d["z_height"], g["z_height"])
+    for k, kr in zip(d["keff"], g["keff"]):
+        assert abs(k - kr) <= REF_TOL, (k, kr)
+    G = int(d["dims"][3])
+    for key in ("fine_flux", "f_psi", "b_psi"):
+        n, _ = flux_errors(d[key], g[key], G=G if key == "fine_flux" else None)
+        print(f"{case} drop-in {key}: vs FP32 reference {n:.2e}")
+        assert n <= REF_TOL, (key, n)

@@ -5,7 +5,8 @@ unmodified reference sources (serial -O0 build, fixed srand seeds).  In its
 reference modes (libc rand() region draws, FP32 sequential tally, FP32
 pairwise reductions) the oracle must reproduce every recorded value bit for
 bit: segment counts, k_eff, residuals, flux after the first sweep, final
-flux, source, z heights and angular flux.
+flux, source, z heights and angular flux (backward too, for the cases
+whose sweeps are the reference's two_way_transport_sweep).
 
 The product's MOC_GEN_REFERENCE generator must reproduce the reference's
 generated problem exactly (sha256 of every array).
@@ -19,7 +20,7 @@ from helpers import GOLDEN, PROBLEM_KEYS, digest, oracle_for
 import simplemoc_amd as sm
 from oracle import pyoracle as po
 
-CASES = ["tiny_quad", "tiny_flat", "tiny_g16_decomp2"]
+CASES = ["tiny_quad", "tiny_flat", "tiny_g16_decomp2", "two_way_quad", "two_way_flat"]
 
 
 def bits_equal(a, b):
@@ -32,7 +33,8 @@ def problem_of(case, golden_index):
     meta = golden_index[case]
     inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
     inp.derive()
-    return meta, sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference")
+    return meta, sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference",
+                                     alloc_b_psi=bool(meta.get("two_way")))
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -54,7 +56,7 @@ def test_oracle_reproduces_reference_bitwise(case, golden_index, libc):
     keff = 1.0
     segs, keffs, ress = [], [], []
     for it in range(meta["iterations"]):
-        segs.append(o.transport_sweep())
+        segs.append(o.two_way_sweep() if meta.get("two_way") else o.transport_sweep())
         if it == 0:
             assert bits_equal(o["fine_flux"], g["flux_after_sweep0"])
             assert bits_equal(o["z_height"], g["z_after_sweep0"])
@@ -65,6 +67,7 @@ def test_oracle_reproduces_reference_bitwise(case, golden_index, libc):
     assert segs == g["segments"].tolist()
     assert bits_equal(np.array(keffs, np.float32), g["keff"])
     assert bits_equal(np.array(ress, np.float32), g["residual"])
-    for k in ("fine_flux", "fine_source", "z_height", "f_psi"):
+    keys = ["fine_flux", "fine_source", "z_height", "f_psi"] + (["b_psi"] if meta.get("two_way") else [])
+    for k in keys:
         assert bits_equal(o[k], g[k]), k
     assert bits_equal(np.float32(o.leakage), g["leakage"][0])
