@@ -22,6 +22,9 @@ Extra objects on the JSON line:
   cpu_baseline: the reference's own OpenMP sweep (reference sources built
                 with the reference Makefile flags, oracle/_ref) timed on this
                 host on a bounded sample of the same workload.
+config also carries the other legs: the other tally mode, the two-way
+sweep on the same workload (two_way), the adjoint against the forward
+sweep (adjoint) and batched against per-problem sweeps (batched).
 """
 from __future__ import annotations
 
@@ -273,6 +276,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched/plain leg")
+    ap.add_argument("--no-two-way", action="store_true", help="skip the two-way sweep leg")
     ap.add_argument("--large", action="store_true",
                     help="only the large shape (~200 GB of HBM): one iteration and one adjoint")
     ap.add_argument("--only", choices=["adjoint", "batched"],
@@ -314,16 +318,19 @@ def main():
         f"({time.perf_counter() - t_setup:.1f} s)")
     G, T = inp.n_egroups, inp.ntracks
 
-    def run_mode(tally, steps, warmup):
-        """Power iterations in one tally mode; returns the measured figures."""
+    def run_mode(tally, steps, warmup, two_way=False):
+        """Power iterations in one tally mode; returns the measured figures.
+        two_way: the sweep is the two-way one (forward and backward pieces
+        both counted; psi read and written once per direction)."""
         opts = sm.Options(tally=tally, lanes_per_track=args.lanes_per_track,
                           pieces_per_batch=args.pieces_per_batch)
         with sm.Solver(prob, opts) as solver:
-            log(f"[{tally}] device context ready, launch {solver.launch()}")
+            log(f"[{tally}{' two-way' if two_way else ''}] device context ready, "
+                f"launch {solver.launch()}")
             keff = 1.0
 
             def step(keff):
-                segs = solver.transport_sweep()
+                segs = solver.two_way_sweep() if two_way else solver.transport_sweep()
                 t = solver.timing()
                 solver.boundary_exchange()
                 solver.renormalize_flux()
@@ -341,7 +348,7 @@ def main():
                 total_int += segs * G
                 att_ms.append(t["attenuate_ms"])
                 sweep_ms.append(t["sweep_ms"])
-                att_bytes.append(24.0 * segs * G + 8.0 * T * G)
+                att_bytes.append(24.0 * segs * G + (16.0 if two_way else 8.0) * T * G)
                 log(f"[{tally}] step {k}: {segs} pieces, keff {keff:.6f}, sweep {t['sweep_ms']:.1f} ms "
                     f"(prep {t['prep_ms']:.1f}, trace {t['trace_ms']:.1f}, attenuate "
                     f"{t['attenuate_ms']:.1f}, finalize {t['finalize_ms']:.1f})")
@@ -397,6 +404,18 @@ def main():
                      "traffic": traffic, "kernel": "k_attenuate", "kernel_ms": main_run["att_ms"]},
         "cpu_baseline": None,
     }
+    if not args.no_two_way:
+        try:
+            r = run_mode(args.tally, 2, 1, two_way=True)
+            line["config"]["two_way"] = {
+                "what": "two_way_transport_sweep (src/solver.c:556-891) on the same workload: "
+                        "forward and backward pieces, both counted",
+                "integrations_per_s": r["value"], "sweep_ms": r["sweep_ms"],
+                "sweep_only_integrations_per_s": r["sweep_rate"], "attenuate_ms": r["att_ms"],
+                "roofline_frac": r["achieved"] / HBM_PEAK_GBS, "keff": r["keff"],
+                "sweep_ms_vs_plain": r["sweep_ms"] / main_run["sweep_ms"]}
+        except Exception as e:  # reported, never fatal to the plain-sweep measurement
+            log(f"two-way leg failed: {e}")
     if not args.no_adjoint:
         try:
             line["config"]["adjoint"] = adjoint_leg(sm)
