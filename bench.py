@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--no-two-way", action="store_true", help="skip the two-way sweep leg")
     ap.add_argument("--large", action="store_true",
                     help="only the large shape (~200 GB of HBM): one iteration and one adjoint")
-    ap.add_argument("--only", choices=["adjoint", "batched"],
+    ap.add_argument("--only", choices=["adjoint", "batched", "two_way"],
                     help="run only this leg and print its figures (for profiling)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only time a plain copy kernel of this many GiB (counter calibration)")
@@ -298,7 +298,7 @@ def main():
 
     import simplemoc_amd as sm
 
-    if args.only:
+    if args.only in ("adjoint", "batched"):
         leg = adjoint_leg(sm) if args.only == "adjoint" else batched_leg(sm, args.tally)
         print(json.dumps({"metric": METRIC, "variant": args.only, "n_gpus": 1,
                           args.only: leg}), flush=True)
@@ -360,6 +360,19 @@ def main():
                     achieved=float(np.mean(att_bytes)) / (att_avg_ms * 1e-3) / 1e9,
                     pieces=int(total_int // (G * steps)))
 
+    def two_way_leg(plain_sweep_ms=None):
+        r = run_mode(args.tally, 2, 1, two_way=True)
+        return {"what": "two_way_transport_sweep (src/solver.c:556-891) on the same workload: "
+                        "forward and backward pieces, both counted",
+                "integrations_per_s": r["value"], "sweep_ms": r["sweep_ms"],
+                "sweep_only_integrations_per_s": r["sweep_rate"], "attenuate_ms": r["att_ms"],
+                "roofline_frac": r["achieved"] / HBM_PEAK_GBS, "keff": r["keff"],
+                "sweep_ms_vs_plain": (r["sweep_ms"] / plain_sweep_ms) if plain_sweep_ms else None}
+
+    if args.only == "two_way":
+        print(json.dumps({"metric": METRIC, "variant": "two_way", "n_gpus": 1,
+                          "two_way": two_way_leg()}), flush=True)
+        return
     main_run = run_mode(args.tally, args.steps, args.warmup)
     other = None
     if not args.single_mode:
@@ -406,14 +419,7 @@ def main():
     }
     if not args.no_two_way:
         try:
-            r = run_mode(args.tally, 2, 1, two_way=True)
-            line["config"]["two_way"] = {
-                "what": "two_way_transport_sweep (src/solver.c:556-891) on the same workload: "
-                        "forward and backward pieces, both counted",
-                "integrations_per_s": r["value"], "sweep_ms": r["sweep_ms"],
-                "sweep_only_integrations_per_s": r["sweep_rate"], "attenuate_ms": r["att_ms"],
-                "roofline_frac": r["achieved"] / HBM_PEAK_GBS, "keff": r["keff"],
-                "sweep_ms_vs_plain": r["sweep_ms"] / main_run["sweep_ms"]}
+            line["config"]["two_way"] = two_way_leg(main_run["sweep_ms"])
         except Exception as e:  # reported, never fatal to the plain-sweep measurement
             log(f"two-way leg failed: {e}")
     if not args.no_adjoint:
