@@ -112,6 +112,44 @@ typedef struct moc_problem {
 #define MOC_SEGS_UNIFORM 1       /* uniform integer in [seg_lo, seg_hi] */
 #define MOC_SEGS_NORMAL 2        /* N(spt, sqrt(spt)) like src/tracks.c:62-65 */
 
+/* ---- OpenMOC track files (src/tracks.c:170-323) ----
+ * Binary, native endianness: i32 string length, the geometry string;
+ * i32 azimuthal angles A, f64 track spacing; i32 tracks[A], i32 num_x[A],
+ * i32 num_y[A], f64 azimuthal weights[A]; then per track (azimuthal angle
+ * by angle) f64 x0, y0, x1, y1, phi, i32 azimuthal index, i32 segment
+ * count, and per segment f64 length, i32 material, i32 region, with two
+ * more i32 (mesh surfaces) per segment in files written with CMFD. */
+typedef struct moc_track_file {
+    char *geometry;              /* NUL-terminated copy of the geometry string */
+    int n_azimuthal;             /* as in the file (not halved) */
+    double spacing;              /* track spacing (cm) */
+    int *num_tracks;             /* [n_azimuthal] */
+    int *num_x;                  /* [n_azimuthal] */
+    int *num_y;                  /* [n_azimuthal] */
+    double *azim_weight;         /* [n_azimuthal] (read; the reference does not use them) */
+    long ntracks_2D;
+    long n_seg2d;
+    double *endpoints;           /* [ntracks_2D][5]: x0, y0, x1, y1, phi */
+    int *azim_index;             /* [ntracks_2D] */
+    long *seg_offset;            /* [ntracks_2D + 1] */
+    float *seg_length;           /* [n_seg2d]: (float) of the file's f64 */
+    int *material_id;            /* [n_seg2d] */
+    long *region_id;             /* [n_seg2d] (Segment.source_id; unused by the sweep,
+                                    as in the reference) */
+} moc_track_file;
+
+/* Read a track file (load_OpenMOC_tracks' parse, with every read, count
+ * and size checked: a short, truncated or inconsistent file is
+ * MOC_ERR_IO / MOC_ERR_INVALID, not a crash). */
+int moc_read_openmoc_tracks(const char *path, int cmfd, moc_track_file **out);
+void moc_track_file_free(moc_track_file *tf);
+/* The input changes loading makes (src/tracks.c:186-189, 209-212,
+ * 311-314): n_azimuthal and radial_ray_sep from the file, ntracks_2D =
+ * the file's tracks, segments_per_track = segments / tracks (integer),
+ * ntracks = ntracks_2D * n_polar_angles * z_stacked.  Needs a derived
+ * input (the reference derives, then loads). */
+int moc_apply_track_file(moc_input *in, const moc_track_file *tf);
+
 typedef struct moc_gen_params {
     int mode;                    /* MOC_GEN_* */
     uint64_t seed;
@@ -121,10 +159,16 @@ typedef struct moc_gen_params {
     float sig_lo, sig_hi;        /* sigma_t U(sig_lo, sig_hi) per region and group */
     int alloc_psi;               /* allocate zeroed f_psi on the host */
     int alloc_b_psi;             /* allocate zeroed b_psi on the host */
+    /* 2D tracks from a track file instead of the generator (the input must
+     * have had moc_apply_track_file with it).  MOC_GEN_REFERENCE then
+     * draws one az_weight per track in file order (src/tracks.c:280) and
+     * no segment counts or lengths, as build_tracks does when loading
+     * (src/init.c:119-123). */
+    const moc_track_file *tracks;
 } moc_gen_params;
 
 /* Defaults: synthetic, seed 1, Poisson, lengths U(0.01,0.5), sigma U(0.1,2.0),
- * no host psi. */
+ * no host psi, no track file. */
 void moc_gen_defaults(moc_gen_params *gp);
 int moc_problem_generate(const moc_input *in, const moc_gen_params *gp, moc_problem **out);
 void moc_problem_free(moc_problem *p);

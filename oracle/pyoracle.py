@@ -340,12 +340,14 @@ def dump_problem(dump: dict):
     return dims, arrays
 
 
-def run_ref_serial(input_file, gen_seed, sweep_seed, iters, dump_path, two_way=False) -> dict:
+def run_ref_serial(input_file, gen_seed, sweep_seed, iters, dump_path, two_way=False,
+                   tracks=None) -> dict:
     """Run the reference harness (serial -O0 build) and read its dump; with
-    two_way the sweep is the reference's two_way_transport_sweep."""
+    two_way the sweep is the reference's two_way_transport_sweep, with
+    tracks (a path) the 2D tracks come from that OpenMOC track file."""
     if not os.path.exists(REF_SERIAL):
         raise FileNotFoundError(REF_SERIAL)
+    extra = (["two_way"] if two_way else []) + ([f"tracks={tracks}"] if tracks else [])
     subprocess.run([REF_SERIAL, str(input_file), str(gen_seed), str(sweep_seed), str(iters),
-                    str(dump_path)] + (["two_way"] if two_way else []), check=True,
-                   stdout=subprocess.DEVNULL)
+                    str(dump_path)] + extra, check=True, stdout=subprocess.DEVNULL)
     return read_dump(dump_path)

@@ -9,12 +9,15 @@
  * seeds, and writes the generated problem and the resulting state to a
  * binary dump that tests/ and tests/golden/ read.
  *
- * Usage: ref_driver <input file> <gen_seed> <sweep_seed> <iters> <dump> [two_way]
+ * Usage: ref_driver <input file> <gen_seed> <sweep_seed> <iters> <dump>
+ *                   [two_way] [tracks=<OpenMOC track file>]
  *
  * With "two_way" each iteration's sweep is the reference's
  * two_way_transport_sweep (src/solver.c:556-891, defined there but not
  * declared in its header) instead of transport_sweep, and b_psi is dumped
- * too.
+ * too.  With "tracks=<file>" build_tracks loads the 2D tracks from the
+ * file (load_OpenMOC_tracks, src/tracks.c:170-323), as the reference's
+ * -d option does (src/io.c:160-168).
  *
  * Dump format: a sequence of records
  *   u32 name_len, name bytes, u32 type (0 f32, 1 i64, 2 i32, 3 f64), u64 count, data
@@ -38,12 +41,22 @@ static void put(const char *name, unsigned type, const void *data, unsigned long
 
 int main(int argc, char *argv[])
 {
-    if (argc != 6 && !(argc == 7 && strcmp(argv[6], "two_way") == 0)) {
-        fprintf(stderr, "usage: %s <input> <gen_seed> <sweep_seed> <iters> <dump> [two_way]\n",
-                argv[0]);
+    int two_way = 0;
+    char *track_file = NULL;
+    int bad = argc < 6;
+    for (int a = 6; a < argc && !bad; a++) {
+        if (strcmp(argv[a], "two_way") == 0)
+            two_way = 1;
+        else if (strncmp(argv[a], "tracks=", 7) == 0)
+            track_file = argv[a] + 7;
+        else
+            bad = 1;
+    }
+    if (bad) {
+        fprintf(stderr, "usage: %s <input> <gen_seed> <sweep_seed> <iters> <dump> [two_way] "
+                "[tracks=<file>]\n", argv[0]);
         return 2;
     }
-    const int two_way = argc == 7;
     unsigned gen_seed = (unsigned)strtoul(argv[2], NULL, 10);
     unsigned sweep_seed = (unsigned)strtoul(argv[3], NULL, 10);
     int iters = atoi(argv[4]);
@@ -53,7 +66,8 @@ int main(int argc, char *argv[])
     Input I = set_default_input();
     read_input_file(&I, argv[1]);
     I.nthreads = 1;
-    I.load_tracks = false;
+    I.load_tracks = track_file != NULL;
+    I.track_file = track_file;
     calculate_derived_inputs(&I);
 
     srand(gen_seed);
@@ -70,7 +84,7 @@ int main(int argc, char *argv[])
 
     long dims[16] = {nt2, I.n_polar_angles, I.z_stacked, G, nsr, fai, I.cai,
                      I.decomp_assemblies_ax, I.axial_exp, T, n_mat,
-                     I.n_azimuthal, params.expTable.N, 0, 0, 0};
+                     I.n_azimuthal, params.expTable.N, I.segments_per_track, 0, 0};
     put("dims", 1, dims, 16);
     float fdims[8] = {I.height, I.axial_z_sep, I.assembly_width, I.domain_height,
                       params.expTable.dx, params.expTable.maxVal, I.precision,

@@ -75,12 +75,24 @@ class _Problem(C.Structure):
     ]
 
 
+class _TrackFile(C.Structure):
+    _fields_ = [
+        ("geometry", C.c_char_p), ("n_azimuthal", C.c_int), ("spacing", C.c_double),
+        ("num_tracks", C.POINTER(C.c_int)), ("num_x", C.POINTER(C.c_int)),
+        ("num_y", C.POINTER(C.c_int)), ("azim_weight", C.POINTER(C.c_double)),
+        ("ntracks_2D", C.c_long), ("n_seg2d", C.c_long), ("endpoints", C.POINTER(C.c_double)),
+        ("azim_index", C.POINTER(C.c_int)), ("seg_offset", C.POINTER(C.c_long)),
+        ("seg_length", _fp), ("material_id", C.POINTER(C.c_int)),
+        ("region_id", C.POINTER(C.c_long)),
+    ]
+
+
 class _GenParams(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("seed", C.c_uint64), ("seg_dist", C.c_int), ("seg_lo", C.c_long),
         ("seg_hi", C.c_long), ("len_lo", C.c_float), ("len_hi", C.c_float),
         ("sig_lo", C.c_float), ("sig_hi", C.c_float), ("alloc_psi", C.c_int),
-        ("alloc_b_psi", C.c_int),
+        ("alloc_b_psi", C.c_int), ("tracks", C.POINTER(_TrackFile)),
     ]
 
 
@@ -132,6 +144,10 @@ def lib():
             "moc_gen_defaults": ([G], None),
             "moc_problem_generate": ([I, G, C.POINTER(P)], C.c_int),
             "moc_problem_free": ([P], None),
+            "moc_read_openmoc_tracks": ([C.c_char_p, C.c_int, C.POINTER(C.POINTER(_TrackFile))],
+                                        C.c_int),
+            "moc_track_file_free": ([C.POINTER(_TrackFile)], None),
+            "moc_apply_track_file": ([I, C.POINTER(_TrackFile)], C.c_int),
             "moc_build_polar_angles": ([C.c_int, _fp], C.c_int),
             "moc_table_size": ([C.c_float, C.c_float], C.c_int),
             "moc_build_table": ([C.c_float, C.c_float, _fp, C.POINTER(C.c_int), _fp, _fp], C.c_int),
@@ -229,6 +245,55 @@ class Input:
     def as_dict(self):
         return {k: getattr(self._s, k) for k, _ in _Input._fields_}
 
+    def apply_track_file(self, tf: "TrackFile"):
+        """What loading a track file changes in the input (src/tracks.c:186-189,
+        311-314): n_azimuthal, radial_ray_sep, ntracks_2D, segments_per_track,
+        ntracks.  Derives the input first if needed."""
+        if not self.derived:
+            self.derive()
+        _check(lib().moc_apply_track_file(C.byref(self._s), tf._p))
+        return self
+
+
+class TrackFile:
+    """An OpenMOC track file (src/tracks.c:170-323), read by the library.
+
+    Arrays are numpy copies: endpoints [ntracks_2D, 5] (x0, y0, x1, y1, phi),
+    azim_index, seg_offset [ntracks_2D + 1], seg_length (float32),
+    material_id, region_id; per azimuthal angle num_tracks, num_x, num_y,
+    azim_weight."""
+
+    def __init__(self, path, cmfd=False):
+        p = C.POINTER(_TrackFile)()
+        _check(lib().moc_read_openmoc_tracks(str(path).encode(), int(bool(cmfd)), C.byref(p)))
+        self._p = p
+        t = p.contents
+        A, nt2, ns = t.n_azimuthal, t.ntracks_2D, t.n_seg2d
+        self.geometry = t.geometry.decode(errors="replace") if t.geometry else ""
+        self.n_azimuthal, self.spacing, self.ntracks_2D, self.n_seg2d = A, t.spacing, nt2, ns
+
+        def arr(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dt, copy=True)
+
+        self.num_tracks = arr(t.num_tracks, A, np.int32)
+        self.num_x = arr(t.num_x, A, np.int32)
+        self.num_y = arr(t.num_y, A, np.int32)
+        self.azim_weight = arr(t.azim_weight, A, np.float64)
+        self.endpoints = arr(t.endpoints, 5 * nt2, np.float64).reshape(nt2, 5)
+        self.azim_index = arr(t.azim_index, nt2, np.int32)
+        self.seg_offset = arr(t.seg_offset, nt2 + 1, np.int64)
+        self.seg_length = arr(t.seg_length, ns, np.float32)
+        self.material_id = arr(t.material_id, ns, np.int32)
+        self.region_id = arr(t.region_id, ns, np.int64)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().moc_track_file_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
 
 class Problem:
     """Flat host arrays of one problem (include/simplemoc.h moc_problem)."""
@@ -250,7 +315,9 @@ class Problem:
     @classmethod
     def generate(cls, inp: Input, seed=1, mode="synthetic", seg_dist="poisson", seg_lo=5,
                  seg_hi=500, len_lo=0.01, len_hi=0.5, sig_lo=0.1, sig_hi=2.0, alloc_psi=False,
-                 alloc_b_psi=False):
+                 alloc_b_psi=False, track_file: "TrackFile | None" = None):
+        """track_file: 2D tracks from an OpenMOC track file (the input must
+        have had apply_track_file with it)."""
         if not inp.derived:
             inp.derive()
         gp = _GenParams()
@@ -261,6 +328,8 @@ class Problem:
         gp.seg_lo, gp.seg_hi = seg_lo, seg_hi
         gp.len_lo, gp.len_hi, gp.sig_lo, gp.sig_hi = len_lo, len_hi, sig_lo, sig_hi
         gp.alloc_psi, gp.alloc_b_psi = int(alloc_psi), int(alloc_b_psi)
+        if track_file is not None:
+            gp.tracks = track_file._p
         out = C.POINTER(_Problem)()
         _check(lib().moc_problem_generate(C.byref(inp._s), C.byref(gp), C.byref(out)))
         return cls(out, True)

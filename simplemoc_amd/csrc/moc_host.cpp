@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -265,6 +266,214 @@ extern "C" void moc_problem_free(moc_problem *p)
     free(p);
 }
 
+// ---------------------------------------------------------------------
+// OpenMOC track files (src/tracks.c:170-323).  One pass over the file; the
+// reference's first pass only counts segments to size one allocation,
+// which the vectors here do by growing.  Every read is checked and every
+// count is bounded by the bytes left in the file before anything is
+// reserved for it.
+namespace {
+struct TrackFileReader {
+    FILE *fp;
+    const char *path;
+    long long left;             // bytes not yet read
+
+    int read(void *dst, size_t size, size_t n, const char *what)
+    {
+        if ((long long)(size * n) > left)
+            return moc_set_error(MOC_ERR_IO, "track file %s ends inside %s", path, what);
+        if (n > 0 && fread(dst, size, n, fp) != n)
+            return moc_set_error(MOC_ERR_IO, "track file %s: read of %s failed", path, what);
+        left -= (long long)(size * n);
+        return MOC_OK;
+    }
+    int skip(long long bytes, const char *what)
+    {
+        if (bytes > left)
+            return moc_set_error(MOC_ERR_IO, "track file %s ends inside %s", path, what);
+        if (fseek(fp, (long)bytes, SEEK_CUR) != 0)
+            return moc_set_error(MOC_ERR_IO, "track file %s: seek past %s failed", path, what);
+        left -= bytes;
+        return MOC_OK;
+    }
+};
+
+template <class T>
+T *copy_out(const std::vector<T> &v)
+{
+    T *p = (T *)malloc((v.size() + 1) * sizeof(T));
+    if (p && !v.empty())
+        memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+}
+
+int parse_track_file(TrackFileReader &r, int cmfd, moc_track_file *tf)
+{
+    int rc;
+    int32_t len = 0;
+    if ((rc = r.read(&len, 4, 1, "the geometry string length")))
+        return rc;
+    if (len < 0)
+        return moc_set_error(MOC_ERR_INVALID, "track file %s: negative geometry string length %d",
+                             r.path, len);
+    if ((long long)len > r.left)
+        return moc_set_error(MOC_ERR_IO, "track file %s ends inside the geometry string", r.path);
+    tf->geometry = (char *)malloc((size_t)len + 1);
+    if (!tf->geometry)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for the geometry string");
+    if ((rc = r.read(tf->geometry, 1, (size_t)len, "the geometry string")))
+        return rc;
+    tf->geometry[len] = '\0';
+    int32_t A = 0;
+    if ((rc = r.read(&A, 4, 1, "the azimuthal angle count")) ||
+        (rc = r.read(&tf->spacing, 8, 1, "the track spacing")))
+        return rc;
+    if (A <= 0)
+        return moc_set_error(MOC_ERR_INVALID, "track file %s: %d azimuthal angles", r.path, A);
+    if ((long long)A * 20 > r.left)
+        return moc_set_error(MOC_ERR_IO, "track file %s ends inside the azimuthal tables", r.path);
+    tf->n_azimuthal = A;
+    tf->num_tracks = (int *)malloc(A * sizeof(int));
+    tf->num_x = (int *)malloc(A * sizeof(int));
+    tf->num_y = (int *)malloc(A * sizeof(int));
+    tf->azim_weight = (double *)malloc(A * sizeof(double));
+    if (!tf->num_tracks || !tf->num_x || !tf->num_y || !tf->azim_weight)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for the azimuthal tables");
+    if ((rc = r.read(tf->num_tracks, 4, A, "the track counts")) ||
+        (rc = r.read(tf->num_x, 4, A, "num_x")) || (rc = r.read(tf->num_y, 4, A, "num_y")) ||
+        (rc = r.read(tf->azim_weight, 8, A, "the azimuthal weights")))
+        return rc;
+    long nt2 = 0;
+    for (int a = 0; a < A; a++) {
+        if (tf->num_tracks[a] < 0)
+            return moc_set_error(MOC_ERR_INVALID, "track file %s: azimuthal angle %d has %d tracks",
+                                 r.path, a, tf->num_tracks[a]);
+        nt2 += tf->num_tracks[a];
+    }
+    // every track holds at least 48 bytes
+    if (nt2 * 48 > r.left)
+        return moc_set_error(MOC_ERR_IO, "track file %s is too short for its %ld tracks", r.path, nt2);
+    tf->ntracks_2D = nt2;
+    const long long seg_bytes = cmfd ? 24 : 16;
+    std::vector<double> ends;
+    std::vector<int> azim, mat;
+    std::vector<long> off(1, 0), reg;
+    std::vector<float> lens;
+    ends.reserve(nt2 * 5);
+    azim.reserve(nt2);
+    off.reserve(nt2 + 1);
+    for (long t = 0; t < nt2; t++) {
+        double e5[5];
+        int32_t ai = 0, ns = 0;
+        if ((rc = r.read(e5, 8, 5, "a track's end points")) ||
+            (rc = r.read(&ai, 4, 1, "a track's azimuthal index")) ||
+            (rc = r.read(&ns, 4, 1, "a track's segment count")))
+            return rc;
+        if (ns < 0)
+            return moc_set_error(MOC_ERR_INVALID, "track file %s: track %ld has %d segments",
+                                 r.path, t, ns);
+        if ((long long)ns * seg_bytes > r.left)
+            return moc_set_error(MOC_ERR_IO, "track file %s ends inside the segments of track %ld",
+                                 r.path, t);
+        ends.insert(ends.end(), e5, e5 + 5);
+        azim.push_back(ai);
+        for (int32_t q = 0; q < ns; q++) {
+            double length = 0.0;
+            int32_t m = 0, g = 0;
+            if ((rc = r.read(&length, 8, 1, "a segment length")) ||
+                (rc = r.read(&m, 4, 1, "a segment material")) ||
+                (rc = r.read(&g, 4, 1, "a segment region")))
+                return rc;
+            if (cmfd && (rc = r.skip(8, "a segment's mesh surfaces")))
+                return rc;
+            lens.push_back((float)length);
+            mat.push_back(m);
+            reg.push_back((long)g);
+        }
+        off.push_back(off.back() + ns);
+    }
+    tf->n_seg2d = off.back();
+    tf->endpoints = copy_out(ends);
+    tf->azim_index = copy_out(azim);
+    tf->seg_offset = copy_out(off);
+    tf->seg_length = copy_out(lens);
+    tf->material_id = copy_out(mat);
+    tf->region_id = copy_out(reg);
+    if (!tf->endpoints || !tf->azim_index || !tf->seg_offset || !tf->seg_length ||
+        !tf->material_id || !tf->region_id)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for the track file arrays");
+    return MOC_OK;
+}
+}  // namespace
+
+extern "C" void moc_track_file_free(moc_track_file *tf)
+{
+    if (!tf)
+        return;
+    free(tf->geometry);
+    free(tf->num_tracks);
+    free(tf->num_x);
+    free(tf->num_y);
+    free(tf->azim_weight);
+    free(tf->endpoints);
+    free(tf->azim_index);
+    free(tf->seg_offset);
+    free(tf->seg_length);
+    free(tf->material_id);
+    free(tf->region_id);
+    free(tf);
+}
+
+extern "C" int moc_read_openmoc_tracks(const char *path, int cmfd, moc_track_file **out)
+{
+    if (!path || !out)
+        return moc_set_error(MOC_ERR_INVALID, "moc_read_openmoc_tracks: null argument");
+    *out = nullptr;
+    FILE *fp = fopen(path, "rb");
+    if (!fp)
+        return moc_set_error(MOC_ERR_IO, "cannot open track file %s: %s", path, strerror(errno));
+    long long size = -1;
+    if (fseek(fp, 0, SEEK_END) == 0) {
+        size = ftell(fp);
+        if (fseek(fp, 0, SEEK_SET) != 0)
+            size = -1;
+    }
+    if (size < 0) {
+        fclose(fp);
+        return moc_set_error(MOC_ERR_IO, "track file %s: cannot determine its size", path);
+    }
+    moc_track_file *tf = (moc_track_file *)calloc(1, sizeof(moc_track_file));
+    if (!tf) {
+        fclose(fp);
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed");
+    }
+    TrackFileReader r{fp, path, size};
+    int rc = parse_track_file(r, cmfd, tf);
+    fclose(fp);
+    if (rc != MOC_OK) {
+        moc_track_file_free(tf);
+        return rc;
+    }
+    *out = tf;
+    return MOC_OK;
+}
+
+extern "C" int moc_apply_track_file(moc_input *in, const moc_track_file *tf)
+{
+    if (!in || !tf)
+        return moc_set_error(MOC_ERR_INVALID, "moc_apply_track_file: null argument");
+    if (!in->derived)
+        return moc_set_error(MOC_ERR_INVALID, "moc_apply_track_file: derive the input first");
+    if (tf->ntracks_2D <= 0)
+        return moc_set_error(MOC_ERR_INVALID, "the track file holds no tracks");
+    in->n_azimuthal = tf->n_azimuthal;
+    in->radial_ray_sep = (float)tf->spacing;
+    in->ntracks_2D = tf->ntracks_2D;
+    in->segments_per_track = tf->n_seg2d / tf->ntracks_2D;
+    in->ntracks = in->ntracks_2D * in->n_polar_angles * in->z_stacked;
+    return MOC_OK;
+}
+
 // --- libc-rand generators of the reference (src/utils.c:4-26) ---
 static float ref_urand(void) { return (float)rand() / (float)RAND_MAX; }
 
@@ -366,16 +575,28 @@ static void init_z(moc_problem *p)
             }
 }
 
-static int gen_reference(moc_problem *p, const moc_gen_params *gp)
+// 2D segments of a track file (src/tracks.c:264-296)
+static int take_file_segments(moc_problem *p, const moc_track_file *tf)
+{
+    if (tf->ntracks_2D != p->in.ntracks_2D)
+        return moc_set_error(MOC_ERR_INVALID, "the input has %ld 2D tracks, the track file %ld "
+                             "(apply the file to the input first)", p->in.ntracks_2D, tf->ntracks_2D);
+    for (long i = 0; i <= tf->ntracks_2D; i++)
+        p->seg_offset[i] = tf->seg_offset[i];
+    p->n_seg2d = tf->n_seg2d;
+    p->seg_length = (float *)malloc((p->n_seg2d + 1) * sizeof(float));
+    if (!p->seg_length)
+        return moc_set_error(MOC_ERR_NOMEM, "host allocation failed for segments");
+    if (p->n_seg2d > 0)
+        memcpy(p->seg_length, tf->seg_length, p->n_seg2d * sizeof(float));
+    return MOC_OK;
+}
+
+// src/tracks.c:29-33, :50-58 and :62-65
+static int ref_segments(moc_problem *p)
 {
     const moc_input &I = p->in;
-    const long nt2 = I.ntracks_2D, G = I.n_egroups, nsr = I.n_source_regions_per_node;
-    const long fai = I.fai;
-    srand((unsigned)gp->seed);
-    // src/tracks.c:11-12
-    for (long i = 0; i < nt2; i++)
-        p->az_weight[i] = ref_urand();
-    // src/tracks.c:29-33 and :62-65
+    const long nt2 = I.ntracks_2D;
     p->seg_offset[0] = 0;
     const float mean = (float)I.segments_per_track;
     const float sigma = (float)sqrt((double)I.segments_per_track);
@@ -395,6 +616,13 @@ static int gen_reference(moc_problem *p, const moc_gen_params *gp)
         for (long n = 0; n < ns; n++)
             p->seg_length[p->seg_offset[i] + n] = ref_urand() * I.assembly_width / (float)ns;
     }
+    return MOC_OK;
+}
+
+static int ref_tracks_and_sources(moc_problem *p)
+{
+    const moc_input &I = p->in;
+    const long G = I.n_egroups, nsr = I.n_source_regions_per_node, fai = I.fai;
     // src/tracks.c:117-132 (z heights and polar weights)
     init_z(p);
     for (long t = 0; t < I.ntracks; t++)
@@ -415,17 +643,11 @@ static int gen_reference(moc_problem *p, const moc_gen_params *gp)
     return MOC_OK;
 }
 
-static int gen_synthetic(moc_problem *p, const moc_gen_params *gp)
+static int synthetic_segments(moc_problem *p, const moc_gen_params *gp)
 {
     const moc_input &I = p->in;
-    const long nt2 = I.ntracks_2D, G = I.n_egroups, nsr = I.n_source_regions_per_node;
-    const long fai = I.fai;
+    const long nt2 = I.ntracks_2D;
     const uint64_t s = gp->seed;
-    if (!(gp->len_hi >= gp->len_lo) || !(gp->len_lo > 0) || !(gp->sig_hi >= gp->sig_lo) ||
-        !(gp->sig_lo > 0))
-        return moc_set_error(MOC_ERR_INVALID, "bad synthetic distribution bounds");
-    for (long i = 0; i < nt2; i++)
-        p->az_weight[i] = gen_u01(s, ST_AZW, (uint64_t)i);
     p->seg_offset[0] = 0;
     for (long i = 0; i < nt2; i++) {
         long ns;
@@ -456,6 +678,23 @@ static int gen_synthetic(moc_problem *p, const moc_gen_params *gp)
     const float dl = gp->len_hi - gp->len_lo;
     for (long e = 0; e < p->n_seg2d; e++)
         p->seg_length[e] = gp->len_lo + dl * gen_u01(s, ST_LEN, (uint64_t)e);
+    return MOC_OK;
+}
+
+static int gen_synthetic(moc_problem *p, const moc_gen_params *gp)
+{
+    const moc_input &I = p->in;
+    const long nt2 = I.ntracks_2D, G = I.n_egroups, nsr = I.n_source_regions_per_node;
+    const long fai = I.fai;
+    const uint64_t s = gp->seed;
+    if (!(gp->len_hi >= gp->len_lo) || !(gp->len_lo > 0) || !(gp->sig_hi >= gp->sig_lo) ||
+        !(gp->sig_lo > 0))
+        return moc_set_error(MOC_ERR_INVALID, "bad synthetic distribution bounds");
+    for (long i = 0; i < nt2; i++)
+        p->az_weight[i] = gen_u01(s, ST_AZW, (uint64_t)i);
+    int rc = gp->tracks ? take_file_segments(p, gp->tracks) : synthetic_segments(p, gp);
+    if (rc != MOC_OK)
+        return rc;
     init_z(p);
     for (long t = 0; t < I.ntracks; t++)
         p->p_weight[t] = gen_u01(s, ST_PW, (uint64_t)t);
@@ -476,6 +715,20 @@ static int gen_synthetic(moc_problem *p, const moc_gen_params *gp)
         p->vol[i] = gen_u01(s, ST_VOL, (uint64_t)i);
     }
     return MOC_OK;
+}
+
+// build_tracks' rand() sequence (src/init.c:106-159)
+static int gen_reference(moc_problem *p, const moc_gen_params *gp)
+{
+    const long nt2 = p->in.ntracks_2D;
+    srand((unsigned)gp->seed);
+    // src/tracks.c:11-12, or one draw per loaded track (src/tracks.c:280)
+    for (long i = 0; i < nt2; i++)
+        p->az_weight[i] = ref_urand();
+    int rc = gp->tracks ? take_file_segments(p, gp->tracks) : ref_segments(p);
+    if (rc != MOC_OK)
+        return rc;
+    return ref_tracks_and_sources(p);
 }
 
 extern "C" int moc_problem_generate(const moc_input *in, const moc_gen_params *gp_in,

@@ -45,6 +45,8 @@ static void usage(void)
     printf("  -t <threads>     Accepted for compatibility\n");
     printf("  -i <filename>    Input file name\n");
     printf("  -s               Small problem flag\n");
+    printf("  -d <filename>    OpenMOC track file (2D tracks)\n");
+    printf("  -w               Two-way sweep (forward, then backward replay)\n");
     printf("  -n <iters>       Power iterations (default 1)\n");
     printf("  -S <seed>        Problem seed (default 1)\n");
     printf("  -g <generator>   synthetic | reference\n");
@@ -85,6 +87,8 @@ int main(int argc, char *argv[])
     moc_default_options(&opt);
     int iters = 1;
     int rc;
+    const char *track_path = NULL;
+    int two_way = 0;
 
     for (int a = 1; a < argc; a++) {
         const char *s = argv[a];
@@ -99,8 +103,11 @@ int main(int argc, char *argv[])
         } else if (!strcmp(s, "-s")) {
             moc_small_input(&in);
         } else if (!strcmp(s, "-d")) {
-            fprintf(stderr, "simplemoc: OpenMOC track files (-d) are not supported\n");
-            return 1;
+            if (++a >= argc)
+                usage();
+            track_path = argv[a];
+        } else if (!strcmp(s, "-w")) {
+            two_way = 1;
         } else if (!strcmp(s, "-n")) {
             if (++a >= argc)
                 usage();
@@ -143,6 +150,19 @@ int main(int argc, char *argv[])
         usage();
     if ((rc = moc_derive_inputs(&in)) != MOC_OK)
         die("deriving the problem shape", rc);
+    /* the reference derives, then loads (src/main.c:23, src/init.c:119-123) */
+    moc_track_file *tracks = NULL;
+    if (track_path) {
+        printf("Reading track data from:\n%s\n", track_path);
+        if ((rc = moc_read_openmoc_tracks(track_path, 0, &tracks)) != MOC_OK)
+            die("reading the track file", rc);
+        if ((rc = moc_apply_track_file(&in, tracks)) != MOC_OK)
+            die("applying the track file", rc);
+        gp.tracks = tracks;
+        printf("Number of 2D tracks = %ld\n", in.ntracks_2D);
+        printf("Number of 3D tracks = %ld\n", in.ntracks);
+        printf("Number of segments = %ld\n", tracks->n_seg2d);
+    }
 
     border();
     center("SimpleMOC transport sweep on AMD Instinct MI355X (gfx950)");
@@ -193,7 +213,7 @@ int main(int argc, char *argv[])
     for (int it = 0; it < iters; it++) {
         long segs = 0;
         double s = now();
-        if ((rc = moc_transport_sweep(ctx, &segs)) != MOC_OK)
+        if ((rc = two_way ? moc_two_way_sweep(ctx, &segs) : moc_transport_sweep(ctx, &segs)) != MOC_OK)
             die("transport sweep", rc);
         double e = now();
         t_transport += e - s;
@@ -250,5 +270,6 @@ int main(int argc, char *argv[])
     border();
     moc_destroy(ctx);
     moc_problem_free(prob);
+    moc_track_file_free(tracks);
     return 0;
 }

@@ -69,12 +69,24 @@ def test_two_way_hash_matches_oracle(G, fai, axial_exp, npol, height):
     np.testing.assert_array_equal(st["z_height"], prob.array("z_height"))
 
 
-def test_two_way_fp32_tally_matches_oracle():
+def test_two_way_fp32_tally_as_accurate_as_serial_fp32():
+    """FP32 atomics accumulate in another order than a serial FP32 sum; both
+    carry FP32 noise.  Against the FP64 build: flux and psi within 1e-5
+    normwise, k_eff within 1e-5 plus three times the serial FP32 sum's own
+    k_eff error (this problem's k_eff grows to ~6 with a residual of ~1e10,
+    which amplifies the tally's rounding)."""
     inp = make_input(n_egroups=32, height=10.0)
     prob = sm.Problem.generate(inp, seed=4, alloc_b_psi=True)
-    out_g, st, out_o, o, _ = run_both(prob, 2, sm.Options(tally="f32"),
-                                      dict(tally=po.TALLY_F32, reduce=po.REDUCE_F64))
-    assert_close(out_g, st, out_o, o, 32, tol=REF_TOL, ktol=REF_TOL)
+    out_g, st, out_64, o64, _ = run_both(prob, 2, sm.Options(tally="f32"))
+    o32 = oracle_for(prob, seed=1, tally=po.TALLY_F32, reduce=po.REDUCE_F64)
+    out_32 = o32.iterate(2, 1.0, two_way=True)
+    assert [x[0] for x in out_g] == [x[0] for x in out_64]
+    np.testing.assert_array_equal(st["z_height"], o64["z_height"])
+    for (_, kg, _), (_, k64, _), (_, k32, _) in zip(out_g, out_64, out_32):
+        assert abs(kg - k64) <= REF_TOL * abs(k64) + 3 * abs(k32 - k64), (kg, k64, k32)
+    for k in ("fine_flux", "fine_source", "f_psi", "b_psi"):
+        norm, _ = flux_errors(st[k], o64[k], G=32 if k in ("fine_flux", "fine_source") else None)
+        assert norm <= REF_TOL, (k, norm)
 
 
 def test_two_way_in_chunks_matches_oracle():

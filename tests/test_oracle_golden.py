@@ -20,7 +20,8 @@ from helpers import GOLDEN, PROBLEM_KEYS, digest, oracle_for
 import simplemoc_amd as sm
 from oracle import pyoracle as po
 
-CASES = ["tiny_quad", "tiny_flat", "tiny_g16_decomp2", "two_way_quad", "two_way_flat"]
+CASES = ["tiny_quad", "tiny_flat", "tiny_g16_decomp2", "two_way_quad", "two_way_flat",
+         "tracks_quad", "tracks_two_way"]
 
 
 def bits_equal(a, b):
@@ -30,11 +31,17 @@ def bits_equal(a, b):
 
 
 def problem_of(case, golden_index):
+    """The case's problem from the product's reference-mode generator (2D
+    tracks from the case's OpenMOC track file where it has one)."""
     meta = golden_index[case]
     inp = sm.Input.from_file(os.path.join(GOLDEN, meta["input"]))
     inp.derive()
+    tf = None
+    if meta.get("tracks"):
+        tf = sm.TrackFile(os.path.join(GOLDEN, meta["tracks"]))
+        inp.apply_track_file(tf)
     return meta, sm.Problem.generate(inp, seed=meta["gen_seed"], mode="reference",
-                                     alloc_b_psi=bool(meta.get("two_way")))
+                                     alloc_b_psi=bool(meta.get("two_way")), track_file=tf)
 
 
 @pytest.mark.parametrize("case", CASES)
