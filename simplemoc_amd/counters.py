@@ -1,0 +1,240 @@
+"""GPU counterparts of the reference's PAPI event sets (src/papi.c:19-134).
+
+The reference selects an event set with the input file's `papi_event_set`
+field (src/io.c:266) or `-p <event>` (src/io.c:150-158), counts it around
+each thread's share of the sweep (src/solver.c:307-314, :535-549) and
+prints the counts and a performance summary (src/papi.c:380-474).  On the
+GPU the same questions are answered by rocprofv3 hardware counters per
+kernel dispatch.  Each set below names the counters that answer the
+reference set's question on gfx950; every set fits one counter pass (at
+most 8 SQ, 4 TCC, 4 TCP, 2 GRBM counters, FETCH_SIZE taking 3 TCC and
+WRITE_SIZE 2), except the bandwidth set, which takes two.
+
+    python -m simplemoc_amd.counters --set 1 [--out DIR] -- <program> [args...]
+
+runs the program under `rocprofv3 --pmc` (one child process per pass; this
+process itself never touches the GPU), then prints, per kernel of the
+sweep, the counts in the reference's "count / symbol / description" layout
+and the reference's summary figure for that set.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import os
+import shutil
+import subprocess
+import sys
+
+# FETCH_SIZE on gfx950 counts half of the bytes a plain device copy of known
+# size reads; WRITE_SIZE counts them exactly (measured with bench.py
+# --copy-probe under each counter)
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
+# SQ_INSTS_VALU_FLOPS_* count per wave instruction (the attenuation's count
+# is ~1.07 per VALU instruction issued, far below one per lane); lanes per
+# wave turn them into floating-point operations when every lane is active
+WAVE_LANES = 64
+
+# set -> (reference name, reference events, [passes: [(counter, description)]])
+EVENT_SETS = {
+    0: ("FLOPS", "PAPI_SP_OPS, PAPI_TOT_CYC", [[
+        ("SQ_INSTS_VALU_FLOPS_FP32", "FP32 VALU floating-point operations"),
+        ("SQ_INSTS_VALU_FLOPS_FP32_TRANS", "FP32 transcendental VALU operations"),
+        ("SQ_INSTS_VALU_FLOPS_FP64", "FP64 VALU floating-point operations"),
+        ("GRBM_GUI_ACTIVE", "GPU busy cycles"),
+    ]]),
+    1: ("Bandwidth", "PAPI_L3_TCM, PAPI_TOT_CYC", [[
+        ("FETCH_SIZE", "KiB fetched from memory past L2"),
+        ("GRBM_GUI_ACTIVE", "GPU busy cycles"),
+    ], [
+        ("WRITE_SIZE", "KiB written to memory past L2 (atomics included)"),
+    ]]),
+    2: ("Stall reasons", "RESOURCE_STALLS:ANY/SB/RS, RESOURCE_STALLS2:OOO_RSRC", [[
+        ("SQ_WAVE_CYCLES", "wave-cycles of resident waves"),
+        ("SQ_WAIT_ANY", "wave-cycles waiting for anything"),
+        ("SQ_WAIT_INST_ANY", "wave-cycles waiting for any instruction issue"),
+        ("SQ_WAIT_INST_LDS", "wave-cycles waiting for LDS instruction issue"),
+        ("SQ_ACTIVE_INST_VMEM", "wave-cycles issuing vector memory instructions"),
+        ("SQ_ACTIVE_INST_VALU", "wave-cycles issuing VALU instructions"),
+    ]]),
+    3: ("Stall percentage", "RESOURCE_STALLS:ANY, PAPI_TOT_CYC", [[
+        ("SQ_WAIT_ANY", "wave-cycles waiting for anything"),
+        ("SQ_WAVE_CYCLES", "wave-cycles of resident waves"),
+    ]]),
+    4: ("Memory loads", "MEM_LOAD_UOPS_RETIRED, :L1_HIT, :L2_HIT, :L3_HIT", [[
+        ("TCP_TOTAL_CACHE_ACCESSES_sum", "vector L1 cache accesses"),
+        ("TCP_TCC_READ_REQ_sum", "vector L1 read misses sent to L2"),
+        ("TCC_HIT_sum", "L2 hits"),
+        ("TCC_MISS_sum", "L2 misses"),
+    ]]),
+    5: ("LLC miss rate", "PAPI_L3_TCM, PAPI_L3_TCA", [[
+        ("TCC_MISS_sum", "L2 misses"),
+        ("TCC_REQ_sum", "L2 requests"),
+    ]]),
+    6: ("Branches", "PAPI_BR_MSP, PAPI_BR_CN, PAPI_BR_PRC", [[
+        ("SQ_INSTS_BRANCH", "branch instructions (no prediction on the GPU)"),
+        ("SQ_INSTS_SALU", "scalar ALU instructions"),
+        ("SQ_INSTS_VALU", "vector ALU instructions"),
+    ]]),
+    7: ("TLB misses", "perf::DTLB-LOADS, -LOAD-MISSES, -STORES, -STORE-MISSES", [[
+        ("TCP_UTCL1_REQUEST_sum", "vector L1 address translation requests"),
+        ("TCP_UTCL1_TRANSLATION_HIT_sum", "UTCL1 translation hits"),
+        ("TCP_UTCL1_TRANSLATION_MISS_sum", "UTCL1 translation misses"),
+        ("TCP_UTCL1_PERMISSION_MISS_sum", "UTCL1 permission misses"),
+    ]]),
+}
+
+# per-pass limits of one rocprofv3 --pmc run on gfx950 (counter block ->
+# hardware counters), and what the derived counters take
+BLOCK_LIMITS = {"SQ": 8, "TCC": 4, "TCP": 4, "TA": 2, "TD": 2, "GRBM": 2}
+DERIVED_COST = {"FETCH_SIZE": {"TCC": 3}, "WRITE_SIZE": {"TCC": 2}}
+
+SWEEP_KERNELS = ("k_attenuate", "k_trace", "k_trace_tw", "k_batt", "k_btrace", "k_adjoint")
+
+
+def pass_cost(counters) -> dict:
+    """Hardware counters per block that one pass of `counters` needs
+    (_sum/_avr/_min/_max of one counter count once)."""
+    seen, cost = set(), collections.Counter()
+    for c in counters:
+        if c in DERIVED_COST:
+            cost.update(DERIVED_COST[c])
+            continue
+        base = c
+        for suf in ("_sum", "_avr", "_min", "_max"):
+            if base.endswith(suf):
+                base = base[: -len(suf)]
+        if base in seen:
+            continue
+        seen.add(base)
+        cost[base.split("_")[0]] += 1
+    return dict(cost)
+
+
+def check_sets():
+    """Raise if a pass asks more of a counter block than one run collects."""
+    for sid, (_, _, passes) in EVENT_SETS.items():
+        for p in passes:
+            for blk, n in pass_cost([c for c, _ in p]).items():
+                if n > BLOCK_LIMITS.get(blk, 0):
+                    raise ValueError(f"event set {sid}: {n} {blk} counters in one pass")
+
+
+def read_counters(path):
+    """{kernel: {"dispatches": n, "ns": total ns, counter: total}} from a
+    rocprofv3 counter_collection.csv."""
+    out = collections.defaultdict(lambda: collections.defaultdict(float))
+    seen = collections.defaultdict(set)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            k = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            k = k.split("<")[0]
+            d = r["Dispatch_Id"]
+            out[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            if d not in seen[k]:
+                seen[k].add(d)
+                out[k]["dispatches"] += 1
+                out[k]["ns"] += float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+    return out
+
+
+def summary(sid, v) -> list:
+    """The reference's PERFORMANCE SUMMARY lines (src/papi.c:457-476) for one
+    kernel's totals v."""
+    s = max(v.get("ns", 0.0), 1.0) * 1e-9
+    pct = lambda a, b: 100.0 * a / b if b else float("nan")  # noqa: E731
+    if sid == 0:
+        fl = sum(v.get(c, 0.0) for c in ("SQ_INSTS_VALU_FLOPS_FP32", "SQ_INSTS_VALU_FLOPS_FP32_TRANS",
+                                         "SQ_INSTS_VALU_FLOPS_FP64"))
+        return [f"GFLOPs: {fl * WAVE_LANES / s / 1e9:.3f} (all {WAVE_LANES} lanes active)"]
+    if sid == 1:
+        b = (v.get("FETCH_SIZE", 0.0) * FETCH_SCALE + v.get("WRITE_SIZE", 0.0) * WRITE_SCALE) * 1024
+        return [f"Bandwidth: {b / s / 1e9:.3f} (GB/s)"]
+    if sid == 2:
+        w = v.get("SQ_WAVE_CYCLES", 0.0)
+        return [f"{'Waiting (any):':30s} {pct(v.get('SQ_WAIT_ANY', 0.0), w):.2f}%",
+                f"{'Waiting for issue:':30s} {pct(v.get('SQ_WAIT_INST_ANY', 0.0), w):.2f}%",
+                f"{'Issuing vector memory:':30s} {pct(v.get('SQ_ACTIVE_INST_VMEM', 0.0), w):.2f}%",
+                f"{'Issuing VALU:':30s} {pct(v.get('SQ_ACTIVE_INST_VALU', 0.0), w):.2f}%"]
+    if sid == 3:
+        return [f"GPU Stalled Wave Cycles: {pct(v.get('SQ_WAIT_ANY', 0.0), v.get('SQ_WAVE_CYCLES', 0.0)):.2f}%"]
+    if sid == 4:
+        acc, miss = v.get("TCP_TOTAL_CACHE_ACCESSES_sum", 0.0), v.get("TCP_TCC_READ_REQ_sum", 0.0)
+        hit, l2m = v.get("TCC_HIT_sum", 0.0), v.get("TCC_MISS_sum", 0.0)
+        return [f"{'L1 hit rate:':30s} {100.0 - pct(miss, acc):.2f}%",
+                f"{'L2 hit rate:':30s} {pct(hit, hit + l2m):.2f}%"]
+    if sid == 5:
+        return [f"L2 miss rate: {pct(v.get('TCC_MISS_sum', 0.0), v.get('TCC_REQ_sum', 0.0)):.2f}%"]
+    if sid == 6:
+        n = v.get("SQ_INSTS_SALU", 0.0) + v.get("SQ_INSTS_VALU", 0.0) + v.get("SQ_INSTS_BRANCH", 0.0)
+        return [f"Branch share of ALU + branch instructions: {pct(v.get('SQ_INSTS_BRANCH', 0.0), n):.2f}%"]
+    if sid == 7:
+        req = v.get("TCP_UTCL1_REQUEST_sum", 0.0)
+        return [f"{'UTCL1 translation miss rate:':30s} "
+                f"{pct(v.get('TCP_UTCL1_TRANSLATION_MISS_sum', 0.0), req):.2f}%",
+                f"{'UTCL1 permission miss rate:':30s} "
+                f"{pct(v.get('TCP_UTCL1_PERMISSION_MISS_sum', 0.0), req):.2f}%"]
+    return []
+
+
+def report(sid, totals, kernels=SWEEP_KERNELS) -> str:
+    name, ref_events, passes = EVENT_SETS[sid]
+    desc = dict(c for p in passes for c in p)
+    lines = ["=" * 79, f"GPU COUNTERS: event set {sid} ({name}; the reference counts {ref_events})",
+             "=" * 79]
+    for k in sorted(totals):
+        if not k.startswith(kernels):
+            continue
+        v = totals[k]
+        lines.append(f"{k}: {int(v['dispatches'])} dispatches, {v['ns'] / 1e6:.3f} ms")
+        lines.append("Count          \tSymbol      \tDescription")
+        for c, d in desc.items():
+            lines.append(f"{v.get(c, 0.0):<15.0f}\t{c}\t{d}")
+        lines += summary(sid, v)
+        lines.append("-" * 79)
+    return "\n".join(lines)
+
+
+def run(sid, cmd, out_dir, timeout=600, rocprof="rocprofv3", collect=True):
+    """Each pass of set `sid` over `cmd` under rocprofv3 --pmc (with collect
+    False, the CSVs a previous run left in out_dir); returns the per-kernel
+    totals of all passes merged."""
+    check_sets()
+    if collect and shutil.which(rocprof) is None:
+        raise RuntimeError(f"{rocprof} not found")
+    totals = collections.defaultdict(lambda: collections.defaultdict(float))
+    for i, p in enumerate(EVENT_SETS[sid][2]):
+        d = os.path.join(out_dir, f"set{sid}_pass{i}")
+        if collect:
+            argv = [rocprof, "--pmc", *[c for c, _ in p], "-d", d, "-o", "run", "--output-format",
+                    "csv", "--", *cmd]
+            subprocess.run(argv, check=True, timeout=timeout)
+        for k, v in read_counters(os.path.join(d, "run_counter_collection.csv")).items():
+            for c, x in v.items():
+                if c in ("dispatches", "ns"):
+                    if i == 0:
+                        totals[k][c] += x
+                else:
+                    totals[k][c] += x
+    return totals
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--set", type=int, required=True, choices=sorted(EVENT_SETS))
+    ap.add_argument("--out", default="gpurun_out/counters")
+    ap.add_argument("--timeout", type=int, default=600)
+    ap.add_argument("--report-only", action="store_true",
+                    help="report the counters a previous run left in --out")
+    ap.add_argument("cmd", nargs=argparse.REMAINDER)
+    a = ap.parse_args(argv)
+    cmd = a.cmd[1:] if a.cmd and a.cmd[0] == "--" else a.cmd
+    if not cmd and not a.report_only:
+        ap.error("give the program to run after --")
+    print(report(a.set, run(a.set, cmd, a.out, a.timeout, collect=not a.report_only)), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
