@@ -203,6 +203,8 @@ typedef struct moc_options {
     int pieces_per_batch;        /* attenuation: pieces whose loads issue together (0 auto; 1, 2, 4) */
     long long max_records;       /* piece records held at once (0: as device memory allows);
                                     a sweep needing more runs in chunks of whole z-stacks */
+    int grad_tally;              /* moc_adjoint's gradient accumulation: MOC_TALLY_F64
+                                    (default) or MOC_TALLY_F32 (half the atomic traffic) */
 } moc_options;
 
 void moc_default_options(moc_options *o);

@@ -101,6 +101,7 @@ class _Options(C.Structure):
         ("qsr_mode", C.c_int), ("seed", C.c_uint64), ("tally", C.c_int), ("exp_mode", C.c_int),
         ("boundary", C.c_int), ("keep_b_psi", C.c_int), ("lanes_per_track", C.c_int),
         ("device", C.c_int), ("pieces_per_batch", C.c_int), ("max_records", C.c_longlong),
+        ("grad_tally", C.c_int),
     ]
 
 
@@ -426,8 +427,11 @@ class Problem:
 
 class Options:
     def __init__(self, qsr="hash", seed=1, tally="f64", exp="table", boundary="reflective",
-                 keep_b_psi=False, lanes_per_track=0, device=0, pieces_per_batch=0, max_records=0):
+                 keep_b_psi=False, lanes_per_track=0, device=0, pieces_per_batch=0, max_records=0,
+                 grad_tally="f64"):
+        """grad_tally: the adjoint's gradient accumulation, "f64" or "f32"."""
         self.qsr, self.seed, self.tally, self.exp = qsr, seed, tally, exp
+        self.grad_tally = grad_tally
         self.boundary, self.keep_b_psi = boundary, keep_b_psi
         self.lanes_per_track, self.device = lanes_per_track, device
         self.pieces_per_batch = pieces_per_batch
@@ -446,6 +450,7 @@ class Options:
         o.device = self.device
         o.pieces_per_batch = self.pieces_per_batch
         o.max_records = self.max_records
+        o.grad_tally = {"f64": TALLY_F64, "f32": TALLY_F32}[self.grad_tally]
         return o
 
 

@@ -157,7 +157,8 @@ struct AdjArgs {
     const float *psi;
     const float *psi_scale;
     const float *w;             // [nfsr][G] functional weights
-    double *acc;                // [nfsr][4][G]: d/dc0, d/dc1, d/dc2, d/dsigma per FSR
+    void *acc;                  // [nfsr][4][G] double or float (G64): d/dc0, d/dc1, d/dc2,
+                                // d/dsigma per FSR
     float *tape;                // [slots][max_pieces][G]
     long long tape_stride;      // max_pieces * G
     double *J;                  // one double
@@ -298,7 +299,7 @@ __device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, 
 
 // lane l of a track owns groups l + r*LPT, r < GPL; PB pieces' loads
 // issue together
-template <int LPT, int GPL, bool QUAD, int EXPM>
+template <int LPT, int GPL, bool QUAD, int EXPM, bool G64>
 __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
 {
     constexpr int PB = ADJ_PB / GPL > 0 ? ADJ_PB / GPL : 1;
@@ -453,13 +454,24 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
                     for (int r = 0; r < GPL; r++) {
                         if (k >= valid)
                             continue;
-                        double *ap = a.acc + (size_t)gf[k] * 4 * G + gg[r];
-                        atomicAdd(ap, (double)g0[k][r]);
-                        if (QUAD) {
-                            atomicAdd(ap + G, (double)g1[k][r]);
-                            atomicAdd(ap + 2 * G, (double)g2[k][r]);
+                        const size_t e = (size_t)gf[k] * 4 * G + gg[r];
+                        if (G64) {
+                            double *ap = (double *)a.acc + e;
+                            atomicAdd(ap, (double)g0[k][r]);
+                            if (QUAD) {
+                                atomicAdd(ap + G, (double)g1[k][r]);
+                                atomicAdd(ap + 2 * G, (double)g2[k][r]);
+                            }
+                            atomicAdd(ap + 3 * G, (double)gsg[k][r]);
+                        } else {
+                            float *ap = (float *)a.acc + e;
+                            atomicAdd(ap, g0[k][r]);
+                            if (QUAD) {
+                                atomicAdd(ap + G, g1[k][r]);
+                                atomicAdd(ap + 2 * G, g2[k][r]);
+                            }
+                            atomicAdd(ap + 3 * G, gsg[k][r]);
                         }
-                        atomicAdd(ap + 3 * G, (double)gsg[k][r]);
                     }
             };
             load(nq - 1);
@@ -510,8 +522,10 @@ __global__ void k_adj_mask(const float *__restrict__ w, uint32_t *__restrict__ m
 }
 
 // acc[fsr][4][G] -> grad_coef[fsr][G][3] and grad_sigma[region][G] (the
-// sigma part summed over the region's fine axial intervals in order)
-__global__ void k_adj_finish(const double *__restrict__ acc, double *__restrict__ gc,
+// sigma part summed over the region's fine axial intervals in order, in
+// double)
+template <class A>
+__global__ void k_adj_finish(const A *__restrict__ acc, double *__restrict__ gc,
                              double *__restrict__ gs, long long nsr, int G, int fai)
 {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nsr * G;
@@ -521,7 +535,7 @@ __global__ void k_adj_finish(const double *__restrict__ acc, double *__restrict_
         double sum = 0.0;
         for (int jf = 0; jf < fai; jf++) {
             const long long fsr = r * fai + jf;
-            const double *ap = acc + fsr * 4 * G + g;
+            const A *ap = acc + fsr * 4 * G + g;
             double *gp = gc + (fsr * G + g) * 3;
             gp[0] = ap[0];
             gp[1] = ap[G];
@@ -975,6 +989,7 @@ extern "C" void moc_default_options(moc_options *o)
     o->device = 0;
     o->pieces_per_batch = 0;
     o->max_records = 0;
+    o->grad_tally = MOC_TALLY_F64;
 }
 
 extern "C" int moc_device_count(int *n)
@@ -1050,6 +1065,10 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
         o = *o_in;
     else
         moc_default_options(&o);
+    if ((o.tally != MOC_TALLY_F64 && o.tally != MOC_TALLY_F32) ||
+        (o.grad_tally != MOC_TALLY_F64 && o.grad_tally != MOC_TALLY_F32))
+        return moc_set_error(MOC_ERR_INVALID, "moc_create: tally and grad_tally are MOC_TALLY_F64 "
+                             "or MOC_TALLY_F32");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return moc_set_error(MOC_ERR_DEVICE, "no HIP device available (this library needs a gfx950 GPU)");
@@ -2081,12 +2100,23 @@ extern "C" int moc_get_coefficients(moc_ctx *c, float *coef3, float *sigma)
 template <int LPT, int GPL, bool QUAD>
 static void launch_adj_lpt(moc_ctx *c, const AdjArgs &a, int grid, size_t lds)
 {
-    if (c->opt.exp_mode == MOC_EXP_TABLE)
-        hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_TABLE>), dim3(grid), dim3(64 * ADJ_WAVES),
-                           lds, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_INTRINSIC>), dim3(grid),
-                           dim3(64 * ADJ_WAVES), lds, c->stream, a);
+    const bool tab = c->opt.exp_mode == MOC_EXP_TABLE;
+    const dim3 blk(64 * ADJ_WAVES);
+    if (c->opt.grad_tally == MOC_TALLY_F32) {
+        if (tab)
+            hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_TABLE, false>), dim3(grid), blk, lds,
+                               c->stream, a);
+        else
+            hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_INTRINSIC, false>), dim3(grid), blk,
+                               lds, c->stream, a);
+    } else {
+        if (tab)
+            hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_TABLE, true>), dim3(grid), blk, lds,
+                               c->stream, a);
+        else
+            hipLaunchKernelGGL((k_adjoint<LPT, GPL, QUAD, MOC_EXP_INTRINSIC, true>), dim3(grid), blk,
+                               lds, c->stream, a);
+    }
 }
 
 template <int LPT, int GPL>
@@ -2238,8 +2268,14 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     }
     if (chunks.size() > 1)
         TRY(release_records(c));
-    hipLaunchKernelGGL(k_adj_finish, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0, c->stream,
-                       c->d_adj_acc, c->d_adj_gc, c->d_adj_gs, (long long)c->nsr, c->G, c->fai);
+    if (c->opt.grad_tally == MOC_TALLY_F32)
+        hipLaunchKernelGGL(k_adj_finish<float>, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0,
+                           c->stream, (const float *)c->d_adj_acc, c->d_adj_gc, c->d_adj_gs,
+                           (long long)c->nsr, c->G, c->fai);
+    else
+        hipLaunchKernelGGL(k_adj_finish<double>, dim3(launch_grid(c->nsr * c->G, 256)), dim3(256), 0,
+                           c->stream, (const double *)c->d_adj_acc, c->d_adj_gc, c->d_adj_gs,
+                           (long long)c->nsr, c->G, c->fai);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[2]));

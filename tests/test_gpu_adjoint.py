@@ -5,7 +5,8 @@ FSRs, tallied by the next sweep.  Parameters: sigma_t per (region, group)
 and the quadratic source coefficients (c0, c1, c2) per (FSR, group).
 
 * gradient vs the oracle's FP64 reverse-mode gradient (table and intrinsic
-  exponential): within 1e-5 of the gradient's scale (the GPU sweep is FP32);
+  exponential): within 1e-5 of the gradient's scale (the GPU sweep is FP32),
+  with the gradients accumulated in FP64 and in FP32 (grad_tally);
 * gradient vs central finite differences (h = 1e-4) of the FP64 oracle
   functional, intrinsic exponential, on a seeded subset of components and
   random directions: within 1e-4 relative (the north star's bar);
@@ -30,7 +31,7 @@ CONFIG4_SMALL = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=2, 
                      height=100.0, precision=0.01, n_2D_source_regions_per_assembly=400)
 
 
-def build(exp="intrinsic", axial_exp=2, seed=4, **over):
+def build(exp="intrinsic", axial_exp=2, seed=4, grad_tally="f64", **over):
     inp = make_input(**dict(CONFIG4_SMALL, axial_exp=axial_exp, **over))
     prob = sm.Problem.generate(inp, seed=seed, alloc_psi=True)
     d = prob.dims
@@ -40,13 +41,16 @@ def build(exp="intrinsic", axial_exp=2, seed=4, **over):
     sel = rng.choice(nfsr, max(1, nfsr // 20), replace=False)
     w = np.zeros((nfsr, G), np.float32)
     w[sel] = np.float32(1.0 / (len(sel) * G))
-    solver = sm.Solver(prob, sm.Options(exp=exp, tally="f64"))
+    solver = sm.Solver(prob, sm.Options(exp=exp, tally="f64", grad_tally=grad_tally))
     return prob, solver, w.ravel(), rng
 
 
-@pytest.mark.parametrize("exp,axial_exp", [("intrinsic", 2), ("table", 2), ("intrinsic", 0)])
-def test_adjoint_matches_fp64_adjoint(exp, axial_exp):
-    prob, solver, w, _ = build(exp, axial_exp)
+@pytest.mark.parametrize("exp,axial_exp,grad_tally", [
+    ("intrinsic", 2, "f64"), ("table", 2, "f64"), ("intrinsic", 0, "f64"),
+    ("intrinsic", 2, "f32"), ("table", 2, "f32"), ("intrinsic", 0, "f32"),
+])
+def test_adjoint_matches_fp64_adjoint(exp, axial_exp, grad_tally):
+    prob, solver, w, _ = build(exp, axial_exp, grad_tally=grad_tally)
     with solver:
         J, gs, gc = solver.adjoint(w)
         coef, sigma = solver.coefficients()
@@ -56,7 +60,7 @@ def test_adjoint_matches_fp64_adjoint(exp, axial_exp):
                                     w.astype(np.float64), exp_mode=emode)
     assert J == pytest.approx(J64, rel=1e-5)
     scale = max(np.abs(gs64).max(), np.abs(gc64).max())
-    print(f"{exp}/{axial_exp}: J {J:.6e}, |grad| max {scale:.3e}, "
+    print(f"{exp}/{axial_exp}/{grad_tally}: J {J:.6e}, |grad| max {scale:.3e}, "
           f"max err sigma {np.abs(gs - gs64).max() / scale:.2e}, coef {np.abs(gc - gc64).max() / scale:.2e}")
     assert np.abs(gs - gs64).max() <= 1e-5 * scale
     assert np.abs(gc - gc64).max() <= 1e-5 * scale
@@ -83,8 +87,9 @@ def test_adjoint_several_groups_per_lane(G):
     assert np.abs(gc - gc64).max() <= 1e-5 * scale
 
 
-def test_adjoint_matches_central_differences():
-    prob, solver, w, rng = build("intrinsic", 2, seed=7)
+@pytest.mark.parametrize("grad_tally", ["f64", "f32"])
+def test_adjoint_matches_central_differences(grad_tally):
+    prob, solver, w, rng = build("intrinsic", 2, seed=7, grad_tally=grad_tally)
     with solver:
         J, gs, gc = solver.adjoint(w)
         coef, sigma = solver.coefficients()
