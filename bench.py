@@ -219,7 +219,8 @@ def batched_leg(sm, tally, iters=20):
 def large_leg(sm, tally, adjoint=True):
     """One power iteration at the large shape (the sweep's piece records
     do not fit beside psi and run in chunks of whole z-stacks), then one
-    adjoint of the mean flux over a seeded 5 % of FSRs."""
+    adjoint of the mean flux over a seeded 5 % of FSRs (gradients
+    accumulated in the tally's precision)."""
     t0 = time.perf_counter()
     inp = sm.Input(**LARGE_WORKLOAD)
     inp.derive()
@@ -230,7 +231,7 @@ def large_leg(sm, tally, adjoint=True):
     out = {"workload": "17x17 assemblies, 400 cm / 2 axial domains, 40 coarse x 5 fine axial "
                        "intervals, 128 azimuthal x 12 polar, 128 groups, Poisson(40) segments",
            "ntracks_3D": T, "psi_bytes": T * G * 4, "tally": tally}
-    with sm.Solver(prob, sm.Options(tally=tally)) as s:
+    with sm.Solver(prob, sm.Options(tally=tally, grad_tally=tally)) as s:
         log(f"large: device context ready ({time.perf_counter() - t0:.1f} s)")
         t1 = time.perf_counter()
         segs = s.transport_sweep()
