@@ -109,3 +109,30 @@ def test_whole_programs_with_track_file():
     print(f"-d tiny.tracks: reference {k_ref:.6f}, drop-in {k_dropin:.6f}, driver {k_drv:.6f}")
     for k in (k_dropin, k_drv):
         assert abs(k - k_ref) <= 2e-6 + REF_TOL * abs(k_ref), (k, k_ref)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="C driver not built")
+def test_driver_two_way_on_loaded_tracks():
+    """The C driver's -w (two-way sweeps) on -d tracks: completes, counts
+    2 x pieces for the quadratic source, and its first k_eff equals the
+    library's own two-way iteration on the same problem (the driver seeds
+    the reference generator with -S and draws by hash with seed 1)."""
+    inp_path = os.path.join(GOLDEN, "tiny_tw.in")
+    tracks = os.path.join(GOLDEN, "tiny.tracks")
+    out = subprocess.run([DRIVER, "-i", inp_path, "-d", tracks, "-g", "reference", "-S", "77", "-w",
+                          "-n", "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    ks = [float(x) for x in re.findall(r"keff = ([-+0-9.eEnaNinf]+)", out.stdout)]
+    segs = [int(x) for x in re.findall(r"3D segments (\d+)", out.stdout)]
+    assert len(ks) == 2 and len(segs) == 2
+    tf = sm.TrackFile(tracks)
+    inp = sm.Input.from_file(inp_path)
+    inp.derive()
+    inp.apply_track_file(tf)
+    prob = sm.Problem.generate(inp, seed=77, mode="reference", track_file=tf)
+    with sm.Solver(prob) as s:
+        lib_out = s.iterate(2, two_way=True)
+    assert segs == [x[0] for x in lib_out]
+    assert segs[0] % 2 == 0
+    for k, (_, kl, _) in zip(ks, lib_out):
+        assert abs(k - kl) <= 1e-6 * max(1.0, abs(kl)), (k, kl)
