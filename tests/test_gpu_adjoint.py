@@ -153,6 +153,6 @@ def test_autograd_function_wrapper():
         st = torch.tensor(sigma, requires_grad=True)
         J = SweepFunctional().apply(ct, st, solver, w)
         (2.0 * J).backward()
-    assert float(J) == pytest.approx(J0, rel=1e-7)
+    assert J.item() == pytest.approx(J0, rel=1e-7)
     np.testing.assert_allclose(ct.grad.numpy(), 2 * gc, rtol=1e-6, atol=1e-12)
     np.testing.assert_allclose(st.grad.numpy(), 2 * gs, rtol=1e-6, atol=1e-12)
