@@ -344,6 +344,8 @@ int moc_batch_destroy(moc_batch *b);
 
 /* Forward angular flux of one track (G floats), renormalisation applied. */
 int moc_get_track_psi(moc_ctx *ctx, long track, float *psi);
+/* Backward angular flux of one track (G floats). */
+int moc_get_track_bpsi(moc_ctx *ctx, long track, float *psi);
 
 /* Diagnostic: mean time of a plain device-to-device copy kernel of `bytes`
  * over `iters` launches (achievable HBM bandwidth; calibration of

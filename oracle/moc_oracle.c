@@ -403,6 +403,158 @@ long orc_transport_sweep(orc_problem *P, orc_opts *o)
 }
 
 /* ------------------------------------------------------------------ */
+/* Per-track piece lists of one stack (the reference's seg_dist/seg_src,
+ * src/solver.c:628-641), grown as needed. */
+typedef struct {
+    int zs;
+    float **dist;
+    long **src;
+    long *cnt;
+    long *size;
+} tw_lists;
+
+static int tw_lists_init(tw_lists *L, int zs)
+{
+    L->zs = zs;
+    L->dist = (float **)calloc(zs, sizeof(float *));
+    L->src = (long **)calloc(zs, sizeof(long *));
+    L->cnt = (long *)calloc(zs, sizeof(long));
+    L->size = (long *)calloc(zs, sizeof(long));
+    if (!L->dist || !L->src || !L->cnt || !L->size)
+        return -1;
+    for (int k = 0; k < zs; k++) {
+        L->size[k] = 64;
+        L->dist[k] = (float *)malloc(64 * sizeof(float));
+        L->src[k] = (long *)malloc(64 * sizeof(long));
+        if (!L->dist[k] || !L->src[k])
+            return -1;
+    }
+    return 0;
+}
+
+static void tw_lists_free(tw_lists *L)
+{
+    for (int k = 0; k < L->zs; k++) {
+        if (L->dist)
+            free(L->dist[k]);
+        if (L->src)
+            free(L->src[k]);
+    }
+    free(L->dist);
+    free(L->src);
+    free(L->cnt);
+    free(L->size);
+}
+
+/* One stack (2D track i, polar angle j) of the two-way sweep,
+ * src/solver.c:632-855: z is the stack's zs heights (reset to their
+ * starting values at the end), fpsi/bpsi its zs*G forward/backward
+ * angular fluxes; pieces tally into flux (FP32) or acc (FP64) unless both
+ * are NULL.  Returns the pieces counted (:782, :827), or -1. */
+static long two_way_stack(const orc_problem *P, orc_opts *o, long i, int j, float *z, float *fpsi,
+                          float *bpsi, float *flux, double *acc, tw_lists *L)
+{
+    const int G = P->G;
+    const double node_dz = (double)(P->height / (float)P->decomp);        /* :560 */
+    const int num_intervals = P->cai * P->fai;
+    const double fine_dz = node_dz / num_intervals;
+    const int zs = P->z_stacked;
+    const float azw = P->az_weight[i];
+    const long off = P->seg_offset[i];
+    const long nseg = P->seg_offset[i + 1] - off;
+    const int pos = j < P->n_polar / 2;                                   /* :634-635 */
+    const float p_angle = P->polar_angle[j];
+    const float mu = (float)cos((double)p_angle);
+    const long t0 = ((long)i * P->n_polar + j) * zs;
+    long total = 0;
+    int begin = 0, end = zs;
+    for (int k = 0; k < zs; k++)
+        L->cnt[k] = 0;
+    for (long n = 0; n < nseg; n++) {
+        const float s_full = (float)(P->seg_length[off + n] / sin((double)p_angle));
+        int completed = 0;
+        for (int k = begin; k < end; k++) {
+            const long t = t0 + k;
+            /* :671 -- the cast binds to z_height alone */
+            int interval = (int)((double)(int)z[k] / fine_dz);
+            float bound;
+            if (pos)
+                bound = (float)((node_dz - (double)z[k]) / (double)mu);
+            else
+                bound = -z[k] / mu;
+            float s;
+            if (s_full < bound) {
+                s = s_full;
+            } else {
+                s = bound;
+                completed++;
+            }
+            int done = 0;
+            int p = 0;
+            while (!done) {
+                float fai_z = pos ? (float)((interval + 1) * fine_dz)
+                                  : (float)(interval * fine_dz);
+                float zd = fai_z - z[k];
+                float sd = zd / mu;
+                float ds, zn;
+                if (sd < s) {
+                    interval += pos ? 1 : -1;
+                    ds = sd;
+                    zn = z[k] + zd;
+                } else {
+                    ds = s;
+                    zn = z[k] + s * mu;
+                }
+                s -= ds;
+                if (s <= 0 || interval < 0 || interval >= num_intervals)
+                    done = 1;
+                long qsr;
+                if (o->qsr_mode == ORC_QSR_LIBC) {
+                    qsr = rand() % P->nsr;
+                } else if (o->qsr_mode == ORC_QSR_STREAM) {
+                    if (!o->qsr_stream || o->qsr_stream_pos >= o->qsr_stream_len)
+                        return -1;
+                    qsr = o->qsr_stream[o->qsr_stream_pos++];
+                } else {
+                    qsr = (long)orc_qsr_hash(o->seed, o->sweep_index, t, n, p, P->nsr);
+                }
+                orc_attenuate(P, o, t, z[k], ds, qsr, mu, azw, fpsi + (long)k * G, flux, acc);
+                if (P->axial_exp == 2)
+                    total++;
+                z[k] = zn;
+                if (L->cnt[k] >= L->size[k]) {
+                    L->size[k] *= 2;
+                    L->dist[k] = (float *)realloc(L->dist[k], L->size[k] * sizeof(float));
+                    L->src[k] = (long *)realloc(L->src[k], L->size[k] * sizeof(long));
+                    if (!L->dist[k] || !L->src[k])
+                        return -1;
+                }
+                L->dist[k][L->cnt[k]] = ds;
+                L->src[k][L->cnt[k]] = qsr;
+                L->cnt[k]++;
+                p++;
+            }
+        }
+        if (pos)
+            end -= completed;
+        else
+            begin += completed;
+    }
+    for (int k = 0; k < zs; k++) {
+        const long t = t0 + k;
+        for (long q = L->cnt[k] - 1; q >= 0; q--) {
+            orc_attenuate(P, o, t, z[k], L->dist[k][q], L->src[k][q], -mu, azw, bpsi + (long)k * G,
+                          flux, acc);
+            if (P->axial_exp == 2)
+                total++;
+            z[k] -= L->dist[k][q] * mu;
+        }
+    }
+    for (int k = 0; k < zs; k++)
+        z[k] = pos ? P->axial_z_sep * (float)k : P->axial_z_sep * (float)(k + 1);
+    return total;
+}
+
 /* src/solver.c:556-891: the two-way sweep.  Per 2D track and polar angle:
  * the stacked tracks are traced forward segment by segment (the interval
  * and boundary arithmetic of :650-755, each piece attenuated into f_psi as
@@ -421,136 +573,28 @@ long orc_two_way_sweep(orc_problem *P, orc_opts *o)
         return -1;
     const int G = P->G;
     const long nval = P->nsr * P->fai * G;
+    const int zs = P->z_stacked;
     double *acc = NULL;
     if (o->tally == ORC_TALLY_F64) {
         acc = (double *)malloc((size_t)nval * sizeof(double));
         for (long e = 0; e < nval; e++)
             acc[e] = (double)P->fine_flux[e];
     }
-    const double node_dz = (double)(P->height / (float)P->decomp);        /* :560 */
-    const int num_intervals = P->cai * P->fai;
-    const double fine_dz = node_dz / num_intervals;
-    const int zs = P->z_stacked;
-    long cap = 64;
-    float **dist = (float **)malloc(zs * sizeof(float *));
-    long **src = (long **)malloc(zs * sizeof(long *));
-    long *cnt = (long *)calloc(zs, sizeof(long));
-    long *size = (long *)malloc(zs * sizeof(long));
-    for (int k = 0; k < zs; k++) {
-        size[k] = cap;
-        dist[k] = (float *)malloc(cap * sizeof(float));
-        src[k] = (long *)malloc(cap * sizeof(long));
-    }
+    tw_lists L;
     long total = 0;
-    int failed = 0;
+    int failed = tw_lists_init(&L, zs) != 0;
     for (long i = 0; i < P->ntracks_2D && !failed; i++) {
-        const float azw = P->az_weight[i];
-        const long off = P->seg_offset[i];
-        const long nseg = P->seg_offset[i + 1] - off;
         for (int j = 0; j < P->n_polar && !failed; j++) {
-            const int pos = j < P->n_polar / 2;                            /* :634-635 */
-            const float p_angle = P->polar_angle[j];
-            const float mu = (float)cos((double)p_angle);
-            float *z = P->z_height + ((long)i * P->n_polar + j) * zs;
             const long t0 = ((long)i * P->n_polar + j) * zs;
-            int begin = 0, end = zs;
-            for (int k = 0; k < zs; k++)
-                cnt[k] = 0;
-            for (long n = 0; n < nseg && !failed; n++) {
-                const float s_full = (float)(P->seg_length[off + n] / sin((double)p_angle));
-                int completed = 0;
-                for (int k = begin; k < end; k++) {
-                    const long t = t0 + k;
-                    /* :671 -- the cast binds to z_height alone */
-                    int interval = (int)((double)(int)z[k] / fine_dz);
-                    float bound;
-                    if (pos)
-                        bound = (float)((node_dz - (double)z[k]) / (double)mu);
-                    else
-                        bound = -z[k] / mu;
-                    float s;
-                    if (s_full < bound) {
-                        s = s_full;
-                    } else {
-                        s = bound;
-                        completed++;
-                    }
-                    int done = 0;
-                    int p = 0;
-                    while (!done) {
-                        float fai_z = pos ? (float)((interval + 1) * fine_dz)
-                                          : (float)(interval * fine_dz);
-                        float zd = fai_z - z[k];
-                        float sd = zd / mu;
-                        float ds, zn;
-                        if (sd < s) {
-                            interval += pos ? 1 : -1;
-                            ds = sd;
-                            zn = z[k] + zd;
-                        } else {
-                            ds = s;
-                            zn = z[k] + s * mu;
-                        }
-                        s -= ds;
-                        if (s <= 0 || interval < 0 || interval >= num_intervals)
-                            done = 1;
-                        long qsr;
-                        if (o->qsr_mode == ORC_QSR_LIBC) {
-                            qsr = rand() % P->nsr;
-                        } else if (o->qsr_mode == ORC_QSR_STREAM) {
-                            if (!o->qsr_stream || o->qsr_stream_pos >= o->qsr_stream_len) {
-                                failed = 1;
-                                break;
-                            }
-                            qsr = o->qsr_stream[o->qsr_stream_pos++];
-                        } else {
-                            qsr = (long)orc_qsr_hash(o->seed, o->sweep_index, t, n, p, P->nsr);
-                        }
-                        orc_attenuate(P, o, t, z[k], ds, qsr, mu, azw, P->f_psi + t * G,
-                                      P->fine_flux, acc);
-                        if (P->axial_exp == 2)
-                            total++;
-                        z[k] = zn;
-                        if (cnt[k] >= size[k]) {
-                            size[k] *= 2;
-                            dist[k] = (float *)realloc(dist[k], size[k] * sizeof(float));
-                            src[k] = (long *)realloc(src[k], size[k] * sizeof(long));
-                        }
-                        dist[k][cnt[k]] = ds;
-                        src[k][cnt[k]] = qsr;
-                        cnt[k]++;
-                        p++;
-                    }
-                    if (failed)
-                        break;
-                }
-                if (pos)
-                    end -= completed;
-                else
-                    begin += completed;
-            }
-            for (int k = 0; k < zs && !failed; k++) {
-                const long t = t0 + k;
-                for (long q = cnt[k] - 1; q >= 0; q--) {
-                    orc_attenuate(P, o, t, z[k], dist[k][q], src[k][q], -mu, azw,
-                                  P->b_psi + t * G, P->fine_flux, acc);
-                    if (P->axial_exp == 2)
-                        total++;
-                    z[k] -= dist[k][q] * mu;
-                }
-            }
-            for (int k = 0; k < zs; k++)
-                z[k] = pos ? P->axial_z_sep * (float)k : P->axial_z_sep * (float)(k + 1);
+            long n = two_way_stack(P, o, i, j, P->z_height + t0, P->f_psi + t0 * G,
+                                   P->b_psi + t0 * G, acc ? NULL : P->fine_flux, acc, &L);
+            if (n < 0)
+                failed = 1;
+            else
+                total += n;
         }
     }
-    for (int k = 0; k < zs; k++) {
-        free(dist[k]);
-        free(src[k]);
-    }
-    free(dist);
-    free(src);
-    free(cnt);
-    free(size);
+    tw_lists_free(&L);
     if (acc) {
         if (!failed)
             for (long e = 0; e < nval; e++)
@@ -559,6 +603,21 @@ long orc_two_way_sweep(orc_problem *P, orc_opts *o)
     }
     o->sweep_index++;
     return failed ? -1 : total;
+}
+
+long orc_two_way_stack(const orc_problem *P, orc_opts *o, long i, int j, float *z, float *fpsi,
+                       float *bpsi)
+{
+    if (P->axial_exp != 0 && P->axial_exp != 2)
+        return -1;
+    if (i < 0 || i >= P->ntracks_2D || j < 0 || j >= P->n_polar)
+        return -1;
+    tw_lists L;
+    long n = -1;
+    if (tw_lists_init(&L, P->z_stacked) == 0)
+        n = two_way_stack(P, o, i, j, z, fpsi, bpsi, NULL, NULL, &L);
+    tw_lists_free(&L);
+    return n;
 }
 
 /* ------------------------------------------------------------------ */

@@ -169,6 +169,11 @@ void orc_attenuate(const orc_problem *P, const orc_opts *o, long track,
 long orc_transport_sweep(orc_problem *P, orc_opts *o);
 /* src/solver.c:556-891 (forward into f_psi, backward into b_psi) */
 long orc_two_way_sweep(orc_problem *P, orc_opts *o);
+/* One stack (i, j) of the two-way sweep without tallies: z[zs] (heights in,
+ * starting heights out), fpsi/bpsi[zs*G] its angular fluxes, updated;
+ * draws as in o at o->sweep_index (not advanced).  Pieces counted, or -1. */
+long orc_two_way_stack(const orc_problem *P, orc_opts *o, long i, int j, float *z, float *fpsi,
+                       float *bpsi);
 void orc_boundary_vacuum(orc_problem *P, const orc_opts *o);
 void orc_renormalize_flux(orc_problem *P, const orc_opts *o);
 float orc_update_sources(orc_problem *P, float keff);

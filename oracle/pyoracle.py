@@ -101,6 +101,9 @@ def lib():
         L.orc_transport_sweep.restype = C.c_long
         L.orc_two_way_sweep.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_two_way_sweep.restype = C.c_long
+        L.orc_two_way_stack.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts), C.c_long,
+                                        C.c_int, _fp, _fp, _fp]
+        L.orc_two_way_stack.restype = C.c_long
         L.orc_boundary_vacuum.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_renormalize_flux.argtypes = [C.POINTER(OrcProblem), C.POINTER(OrcOpts)]
         L.orc_update_sources.argtypes = [C.POINTER(OrcProblem), C.c_float]
@@ -296,6 +299,27 @@ class Oracle:
                        "itemsize": C.sizeof(OrcPiece)})
         rec = np.frombuffer(buf, dtype=dt, count=n).copy()
         return rec, z
+
+
+    def two_way_stack(self, i: int, j: int, z_state, f_psi, b_psi, sweep_index: int):
+        """One stack of the two-way sweep without tallies (src/solver.c:632-855),
+        draws of sweep `sweep_index`: returns (pieces, z, f_psi, b_psi) from
+        copies of the given heights [zs] and angular fluxes [zs*G]."""
+        zs, G = self.d["z_stacked"], self.d["G"]
+        z = np.array(z_state, np.float32, copy=True).ravel()
+        fp = np.array(f_psi, np.float32, copy=True).ravel()
+        bp = np.array(b_psi, np.float32, copy=True).ravel()
+        assert z.size == zs and fp.size == zs * G and bp.size == zs * G
+        saved = self.opts.sweep_index
+        self.opts.sweep_index = sweep_index
+        try:
+            n = lib().orc_two_way_stack(C.byref(self.p), C.byref(self.opts), i, j,
+                                        _ptr(z, C.c_float), _ptr(fp, C.c_float), _ptr(bp, C.c_float))
+        finally:
+            self.opts.sweep_index = saved
+        if n < 0:
+            raise RuntimeError("oracle two-way stack failed")
+        return int(n), z, fp, bp
 
 
 # --------------------------------------------------------------------------

@@ -168,6 +168,7 @@ def lib():
             "moc_get_track_records": ([C.c_void_p, C.c_long, _fp, _fp, C.POINTER(C.c_uint32),
                                        C.c_long, C.POINTER(C.c_long)], C.c_int),
             "moc_get_track_psi": ([C.c_void_p, C.c_long, _fp], C.c_int),
+            "moc_get_track_bpsi": ([C.c_void_p, C.c_long, _fp], C.c_int),
             "moc_adjoint": ([C.c_void_p, _fp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                              C.POINTER(C.c_double)], C.c_int),
             "moc_set_coefficients": ([C.c_void_p, _fp, _fp], C.c_int),
@@ -597,10 +598,12 @@ class Solver:
         _check(lib().moc_get_coefficients(self._h, c.ctypes.data_as(_fp), sg.ctypes.data_as(_fp)))
         return c.reshape(-1, 3), sg
 
-    def track_psi(self, track: int):
+    def track_psi(self, track: int, backward: bool = False):
+        """Forward (or backward) angular flux of one track."""
         G = self.problem._p.contents.inp.n_egroups
         out = np.zeros(G, np.float32)
-        _check(lib().moc_get_track_psi(self._h, track, out.ctypes.data_as(_fp)))
+        fn = lib().moc_get_track_bpsi if backward else lib().moc_get_track_psi
+        _check(fn(self._h, track, out.ctypes.data_as(_fp)))
         return out
 
     def track_records(self, track: int, cap: int = 1 << 16):

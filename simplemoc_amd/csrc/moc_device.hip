@@ -2051,6 +2051,24 @@ extern "C" int moc_get_track_psi(moc_ctx *c, long track, float *psi_out)
     return MOC_OK;
 }
 
+extern "C" int moc_get_track_bpsi(moc_ctx *c, long track, float *psi_out)
+{
+    if (!c || !psi_out || track < 0 || track >= c->T)
+        return moc_set_error(MOC_ERR_INVALID, "moc_get_track_bpsi: bad argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->d_bpsi) {
+        // never swept: zero, or NaN after a non-finite normalisation factor
+        float flag = 0.0f;
+        HIP_TRY(hipMemcpy(&flag, c->d_scal + SC_BPSI_NAN, 4, hipMemcpyDeviceToHost));
+        for (int g = 0; g < c->G; g++)
+            psi_out[g] = flag != 0.0f ? NAN : 0.0f;
+        return MOC_OK;
+    }
+    HIP_TRY(hipMemcpy(psi_out, c->d_bpsi + (size_t)track * c->G, (size_t)c->G * 4, hipMemcpyDeviceToHost));
+    return MOC_OK;
+}
+
 // ---------------------------------------------------------------------
 // caller-supplied coefficients, and the adjoint
 extern "C" int moc_set_coefficients(moc_ctx *c, const float *coef3, const float *sigma)

@@ -5,6 +5,10 @@
   oracle's attenuation recurrence applied to the oracle's own trace of the
   same stack (psi does not depend on the tally order, so this is exact),
   in both tally modes;
+* config-2 shape, two-way sweep: after one sweep the forward and backward
+  angular flux of sampled tracks equal, bit for bit, the oracle's two-way
+  pass over the same stack (src/solver.c:632-855), and z is back at its
+  starting heights;
 * config-1 shape in full (4,846 2D tracks, G=8): two power iterations
   against the oracle in the same modes.
 """
@@ -51,6 +55,36 @@ def test_config2_sampled_track_psi_bitwise(tally):
                 got = s.track_psi(t)
                 np.testing.assert_array_equal(got, psi)
                 checked += 1
+    assert checked == 15
+
+
+def test_config2_two_way_sampled_tracks_bitwise():
+    inp = make_input(**CONFIG2)
+    prob = sm.Problem.generate(inp, seed=2)
+    d = prob.dims
+    G, zs, npol = d["G"], d["z_stacked"], d["n_polar"]
+    o = oracle_for(prob, psi=False)
+    z0 = prob.array("z_height").copy()
+    rng = np.random.default_rng(13)
+    zero = np.zeros(zs * G, np.float32)
+    with sm.Solver(prob, sm.Options(tally="f32")) as s:
+        segs = s.two_way_sweep()
+        assert segs > 0 and segs % 2 == 0
+        checked = 0
+        for _ in range(5):
+            i = int(rng.integers(d["ntracks_2D"]))
+            j = int(rng.integers(npol))
+            base = (i * npol + j) * zs
+            n, z_end, fpsi, bpsi = o.two_way_stack(i, j, z0[base:base + zs], zero, zero, 0)
+            assert n > 0
+            np.testing.assert_array_equal(z_end, z0[base:base + zs])
+            for k in rng.integers(zs, size=3):
+                t = base + int(k)
+                np.testing.assert_array_equal(s.track_psi(t), fpsi[k * G:(k + 1) * G])
+                np.testing.assert_array_equal(s.track_psi(t, backward=True), bpsi[k * G:(k + 1) * G])
+                checked += 1
+        st = s.state(psi=False)
+    np.testing.assert_array_equal(st["z_height"], z0)
     assert checked == 15
 
 
