@@ -1120,6 +1120,9 @@ static int create_context(const moc_problem *p, const moc_options *o_in, moc_ctx
     if (c->fai > MAX_FAI || c->G > 16 << PW_GROUP_DEPTH)
         return fail(moc_set_error(MOC_ERR_INVALID, "%d fine intervals and %d groups exceed the "
                                   "supported %d and %d", c->fai, c->G, MAX_FAI, 16 << PW_GROUP_DEPTH));
+    // one trace workgroup holds a z-stack: at most 8 tracks per thread, 256 threads
+    if (c->zs < 1 || c->zs > 2048)
+        return fail(moc_set_error(MOC_ERR_INVALID, "z stacks of %d tracks (1 to 2048 supported)", c->zs));
     if ((size_t)c->table_N * sizeof(float2) > 65536)
         return fail(moc_set_error(MOC_ERR_INVALID, "exponential table of %d cells exceeds 64 KiB of LDS",
                                   c->table_N));
