@@ -134,6 +134,7 @@ struct EmitCtx {
     float dz_att;
     unsigned *err;
     const uint32_t *wmask;      // FSRs to watch for SegResult::last_weighted (or null)
+    int quiet_cap;              // records beyond rec_cap are dropped without an error
 };
 
 // {ds, zin, fsr} of a piece of region qsr starting at height zk, at record
@@ -158,7 +159,7 @@ __device__ __forceinline__ uint32_t store_record(const EmitCtx *ec, long long q,
     const uint32_t fsr = qsr * (uint32_t)ec->fai + (uint32_t)fine;
     if (q < ec->rec_cap)
         ec->rec[q] = make_float4(ds, zin, __uint_as_float(fsr), 0.f);
-    else
+    else if (!ec->quiet_cap)
         atomicOr(ec->err, ERR_RECORD_CAP);
     return fsr;
 }
@@ -297,6 +298,15 @@ enum : int {
 // not, the track's count does not move and later segments write over
 // those records.  Every write stays inside the track's own region because
 // a segment traced at any z has at most its bound's pieces.
+//
+// TRACE_WRITE with hash draws does the same into the track's exact region
+// [off[t], off[t+1]) from the count pass: the pieces of the segments the
+// serial loop does reach are exactly that many, so they always fit; a
+// speculative trace of a segment it does not reach may run past the end,
+// and those writes are dropped (the segment is traced again later).  With
+// explicit draws a piece's draw depends on the pieces before it in the
+// serial order, so the reached segments are traced a second time to emit.
+// Either way each track's final count must equal its region's size.
 template <int MODE, int KPT>
 __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uint64_t skey, int *s_w)
 {
@@ -304,6 +314,8 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
     constexpr bool EMIT = MODE == TRACE_EMIT;
     const int zs = a.zs;
     const int k0 = threadIdx.x * KPT;
+    // records emitted in the first trace of each segment
+    const bool spec = EMIT || (WRITE && !a.qsr_stream);
     {
         const long long i = s / a.np;
         const int j = (int)(s - i * a.np);
@@ -323,7 +335,7 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
             cnt[r] = 0;
             lastw[r] = -1;
             base[r] = ((WRITE || EMIT) && k < zs) ? a.off[t0 + k] - a.rec_base : 0;
-            cap[r] = (EMIT && k < zs) ? a.off[t0 + k + 1] - a.rec_base : a.rec_cap;
+            cap[r] = (spec && k < zs) ? a.off[t0 + k + 1] - a.rec_base : a.rec_cap;
             tkey[r] = ((WRITE || EMIT) && !a.qsr_stream) ? track_key(skey, t0 + k) : 0;
         }
         int begin = 0, end = zs;
@@ -340,7 +352,7 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
             for (int r = 0; r < KPT; r++) {
                 const int k = k0 + r;
                 act[r] = (k >= begin && k < end);
-                if (act[r] && EMIT) {
+                if (act[r] && spec) {
                     EmitCtx ec;
                     ec.rec = a.rec;
                     ec.rec_cap = cap[r];
@@ -354,6 +366,7 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
                     ec.dz_att = a.dz_att;
                     ec.err = a.err;
                     ec.wmask = a.wmask;
+                    ec.quiet_cap = WRITE;
                     res[r] = trace_segment<true>(z[r], s_full, k, pos, last, n, a, c, &ec);
                 } else if (act[r]) {
                     res[r] = trace_segment<false>(z[r], s_full, k, pos, last, n, a, c, nullptr);
@@ -414,7 +427,7 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
             for (int r = 0; r < KPT; r++) {
                 if (!proc[r])
                     continue;
-                if (WRITE) {
+                if (WRITE && !spec) {
                     EmitCtx ec;
                     ec.rec = a.rec;
                     ec.rec_cap = a.rec_cap;
@@ -428,12 +441,13 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
                     ec.dz_att = a.dz_att;
                     ec.err = a.err;
                     ec.wmask = a.wmask;
+                    ec.quiet_cap = 0;
                     const SegResult e = trace_segment<true>(z[r], s_full, k0 + r, pos, last, n, a, c, &ec);
                     if (e.last_weighted >= 0)
                         lastw[r] = cnt[r] + e.last_weighted;
                     ser += res[r].pieces;
                 }
-                if (EMIT && res[r].last_weighted >= 0)
+                if (spec && res[r].last_weighted >= 0)
                     lastw[r] = cnt[r] + res[r].last_weighted;
                 cnt[r] += res[r].pieces;
                 z[r] = res[r].zfin;
@@ -444,6 +458,8 @@ __device__ __forceinline__ void trace_stack(const TraceArgs &a, long long s, uin
         for (int r = 0; r < KPT; r++) {
             const int k = k0 + r;
             if (k < zs) {
+                if (WRITE && (long long)cnt[r] != a.off[t0 + k + 1] - a.off[t0 + k])
+                    atomicOr(a.err, ERR_RECORD_CAP);
                 if ((WRITE || EMIT) && a.commit)
                     a.z[t0 + k] = z[r];
                 if (EMIT || !(WRITE && a.commit))
@@ -582,6 +598,7 @@ __device__ __forceinline__ void trace_stack_tw(const TraceArgs &a, long long s, 
     ec.dz_att = a.dz_att;
     ec.err = a.err;
     ec.wmask = nullptr;
+    ec.quiet_cap = 0;
     // hash draws are emitted in the first trace; explicit draws need the
     // serial position of each track's first piece, known after a scan
     const bool direct = WRITE && !a.qsr_stream;
