@@ -297,7 +297,7 @@ def main():
                           "gb_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); this build runs "
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the library runs "
               "the sweep on a single MI355X and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 

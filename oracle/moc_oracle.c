@@ -1,7 +1,7 @@
 /*
  * moc_oracle.c -- CPU restatement of SimpleMOC's transport-sweep path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, smoke() and
+ * TEST INFRASTRUCTURE ONLY (the oracle): loaded by tests/, smoke() and
  * bench.py's cpu_baseline leg; never by the product.
  *
  * Build: plain C99, -O2 -ffp-contract=off, no -ffast-math, so float and

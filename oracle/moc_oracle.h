@@ -1,7 +1,7 @@
 /*
  * moc_oracle.h -- CPU restatement of SimpleMOC's transport-sweep path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle that the HIP
  * implementation in simplemoc_amd/ is compared against.  It may be loaded
  * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
  * and by nothing else.  The product never links, loads or calls it.

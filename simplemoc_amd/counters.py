@@ -224,7 +224,7 @@ def run(sid, cmd, out_dir, timeout=600, rocprof="rocprofv3", collect=True):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--set", type=int, required=True, choices=sorted(EVENT_SETS))
-    ap.add_argument("--out", default="gpurun_out/counters")
+    ap.add_argument("--out", default="counters_out")
     ap.add_argument("--timeout", type=int, default=600)
     ap.add_argument("--report-only", action="store_true",
                     help="report the counters a previous run left in --out")

@@ -35,15 +35,19 @@ def make_input(**kw) -> sm.Input:
     return inp
 
 
-# Named workloads (the shapes of the benchmark configurations)
-CONFIG1 = dict(x_assemblies=2, y_assemblies=2, cai=2, fai=5, axial_exp=2, radial_ray_sep=0.05,
-               axial_z_sep=0.25, n_azimuthal=16, n_polar_angles=4, n_egroups=8, decompose=1,
-               decomp_assemblies_ax=1, segments_per_track=20, assembly_width=21.42, height=20.0,
-               precision=0.01, n_2D_source_regions_per_assembly=5000)
-CONFIG2 = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2, radial_ray_sep=0.05,
-               axial_z_sep=0.25, n_azimuthal=64, n_polar_angles=10, n_egroups=128, decompose=1,
-               decomp_assemblies_ax=20, segments_per_track=40, assembly_width=21.42,
-               height=400.0, precision=0.01, n_2D_source_regions_per_assembly=5000)
+# Named workloads
+# small core: 2x2 assemblies, 20 cm, 2 coarse x 5 fine intervals, 16 azimuthal x 4 polar, 8 groups
+SMALL_CORE = dict(x_assemblies=2, y_assemblies=2, cai=2, fai=5, axial_exp=2, radial_ray_sep=0.05,
+                  axial_z_sep=0.25, n_azimuthal=16, n_polar_angles=4, n_egroups=8, decompose=1,
+                  decomp_assemblies_ax=1, segments_per_track=20, assembly_width=21.42, height=20.0,
+                  precision=0.01, n_2D_source_regions_per_assembly=5000)
+# one axial domain of a full core: 17x17 assemblies, 400 cm / 20 domains, 20 coarse x 5
+# fine intervals, 64 azimuthal x 10 polar, 128 groups
+FULL_CORE_DOMAIN = dict(x_assemblies=17, y_assemblies=17, cai=20, fai=5, axial_exp=2,
+                        radial_ray_sep=0.05, axial_z_sep=0.25, n_azimuthal=64, n_polar_angles=10,
+                        n_egroups=128, decompose=1, decomp_assemblies_ax=20, segments_per_track=40,
+                        assembly_width=21.42, height=400.0, precision=0.01,
+                        n_2D_source_regions_per_assembly=5000)
 
 
 def oracle_for(problem: sm.Problem, psi: bool = True, **kw):

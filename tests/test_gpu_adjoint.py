@@ -9,11 +9,11 @@ and the quadratic source coefficients (c0, c1, c2) per (FSR, group).
   with the gradients accumulated in FP64 and in FP32 (grad_tally);
 * gradient vs central finite differences (h = 1e-4) of the FP64 oracle
   functional, intrinsic exponential, on a seeded subset of components and
-  random directions: within 1e-4 relative (the north star's bar);
+  random directions: within 1e-4 relative (the required accuracy);
 * J equals the weighted tally of an actual FP64-tally sweep;
 * the call leaves the solver state untouched;
 * the torch.autograd.Function wrapper returns the same J and gradients.
-Shape: the config-4 layout (32 azimuthal x 6 polar, 32 groups, 100 cm,
+Shape: the adjoint workload's layout (32 azimuthal x 6 polar, 32 groups, 100 cm,
 5 coarse x 5 fine intervals, 4 axial domains) with fewer radial tracks.
 """
 import numpy as np
@@ -25,14 +25,14 @@ from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
-CONFIG4_SMALL = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=2, radial_ray_sep=4.0,
+ADJOINT_SHAPE_SMALL = dict(x_assemblies=4, y_assemblies=4, cai=5, fai=5, axial_exp=2, radial_ray_sep=4.0,
                      axial_z_sep=0.25, n_azimuthal=32, n_polar_angles=6, n_egroups=32,
                      decomp_assemblies_ax=4, segments_per_track=20, assembly_width=21.42,
                      height=100.0, precision=0.01, n_2D_source_regions_per_assembly=400)
 
 
 def build(exp="intrinsic", axial_exp=2, seed=4, grad_tally="f64", **over):
-    inp = make_input(**dict(CONFIG4_SMALL, axial_exp=axial_exp, **over))
+    inp = make_input(**dict(ADJOINT_SHAPE_SMALL, axial_exp=axial_exp, **over))
     prob = sm.Problem.generate(inp, seed=seed, alloc_psi=True)
     d = prob.dims
     rng = np.random.default_rng(seed)

@@ -1,15 +1,16 @@
-"""GPU parity at the benchmark configurations' full sizes.
+"""GPU parity at the benchmark workloads' full sizes.
 
-* config-2 shape (19,386 2D tracks x 10 polar x 80 stacked, G=128): after
+* one domain of a full core (17x17 assemblies, 400 cm / 20 axial domains;
+  19,386 2D tracks x 10 polar x 80 stacked, G=128): after
   one sweep the angular flux of sampled tracks equals, bit for bit, the
   oracle's attenuation recurrence applied to the oracle's own trace of the
   same stack (psi does not depend on the tally order, so this is exact),
   in both tally modes;
-* config-2 shape, two-way sweep: after one sweep the forward and backward
+* the same shape, two-way sweep: after one sweep the forward and backward
   angular flux of sampled tracks equal, bit for bit, the oracle's two-way
   pass over the same stack (src/solver.c:632-855), and z is back at its
   starting heights;
-* config-1 shape in full (4,846 2D tracks, G=8): two power iterations
+* the small core in full (2x2 assemblies, 20 cm; 4,846 2D tracks, G=8): two power iterations
   against the oracle in the same modes.
 """
 import ctypes as C
@@ -19,15 +20,15 @@ import numpy as np
 import pytest
 
 import simplemoc_amd as sm
-from helpers import CONFIG1, CONFIG2, flux_errors, make_input, oracle_for
+from helpers import SMALL_CORE, FULL_CORE_DOMAIN, flux_errors, make_input, oracle_for
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("tally", ["f32", "f64"])
-def test_config2_sampled_track_psi_bitwise(tally):
-    inp = make_input(**CONFIG2)
+def test_full_core_domain_sampled_track_psi_bitwise(tally):
+    inp = make_input(**FULL_CORE_DOMAIN)
     prob = sm.Problem.generate(inp, seed=2)
     d = prob.dims
     G, zs, npol = d["G"], d["z_stacked"], d["n_polar"]
@@ -58,8 +59,8 @@ def test_config2_sampled_track_psi_bitwise(tally):
     assert checked == 15
 
 
-def test_config2_two_way_sampled_tracks_bitwise():
-    inp = make_input(**CONFIG2)
+def test_full_core_domain_two_way_sampled_tracks_bitwise():
+    inp = make_input(**FULL_CORE_DOMAIN)
     prob = sm.Problem.generate(inp, seed=2)
     d = prob.dims
     G, zs, npol = d["G"], d["z_stacked"], d["n_polar"]
@@ -88,8 +89,8 @@ def test_config2_two_way_sampled_tracks_bitwise():
     assert checked == 15
 
 
-def test_config1_full_shape_matches_oracle():
-    inp = make_input(**CONFIG1)
+def test_small_core_full_shape_matches_oracle():
+    inp = make_input(**SMALL_CORE)
     prob = sm.Problem.generate(inp, seed=1)
     o = oracle_for(prob)
     with sm.Solver(prob, sm.Options(tally="f64")) as s:

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import simplemoc_amd as sm
-from helpers import CONFIG1, CONFIG2, GOLDEN, flux_errors, make_input, oracle_for
+from helpers import SMALL_CORE, FULL_CORE_DOMAIN, GOLDEN, flux_errors, make_input, oracle_for
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -57,9 +57,9 @@ def test_hash_mode_matches_oracle(G, fai, axial_exp, npol):
     assert_close(out_g, st, out_o, o, G)
 
 
-def test_config1_shape_reduced_matches_oracle():
-    # the config-1 shape with 1/10 of the radial tracks (radial sep 0.5)
-    inp = make_input(**{**CONFIG1, "radial_ray_sep": 0.5})
+def test_small_core_reduced_matches_oracle():
+    # the small core with 1/10 of the radial tracks (radial sep 0.5)
+    inp = make_input(**{**SMALL_CORE, "radial_ray_sep": 0.5})
     prob = sm.Problem.generate(inp, seed=1)
     out_g, st, out_o, o = run_both(prob, 3)
     assert_close(out_g, st, out_o, o, 8)
@@ -209,11 +209,11 @@ def test_fp32_tally_as_accurate_as_reference(case, golden_index, libc):
     assert e_gpu.max() <= 10 * e_ref.max() + NEAR
 
 
-def test_trace_records_match_oracle_at_config2_shape():
-    """Size-independent check at the full config-2 shape: the device trace of
+def test_trace_records_match_oracle_at_full_core_domain_shape():
+    """Size-independent check at the full-core domain shape: the device trace of
     sampled tracks (piece lengths, in-cell z, FSR index) equals the oracle's
     trace of the same stacks from the same z state."""
-    inp = make_input(**CONFIG2)
+    inp = make_input(**FULL_CORE_DOMAIN)
     prob = sm.Problem.generate(inp, seed=2)
     d = prob.dims
     o = oracle_for(prob, psi=False)

@@ -97,7 +97,7 @@ def keff_of(cmd, env=None):
 def test_whole_programs_with_track_file():
     """-d <file> through the reference's main, the relinked main and the C
     driver.  The reference's main seeds rand() with time(NULL) = 1000
-    (pinned by oracle/time_shim.c) and runs one iteration; the driver
+    (pinned by oracle/time_shim.c) and runs one iteration; the C driver
     gets the same seed for the reference generator and the same draws."""
     inp = os.path.join(GOLDEN, "tiny_tw.in")
     tracks = os.path.join(GOLDEN, "tiny.tracks")
@@ -115,7 +115,7 @@ def test_whole_programs_with_track_file():
 def test_driver_two_way_on_loaded_tracks():
     """The C driver's -w (two-way sweeps) on -d tracks: completes, counts
     2 x pieces for the quadratic source, and its first k_eff equals the
-    library's own two-way iteration on the same problem (the driver seeds
+    library's own two-way iteration on the same problem (the C driver seeds
     the reference generator with -S and draws by hash with seed 1)."""
     inp_path = os.path.join(GOLDEN, "tiny_tw.in")
     tracks = os.path.join(GOLDEN, "tiny.tracks")

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import simplemoc_amd as sm
-from helpers import CONFIG2, GOLDEN, ROOT, make_input
+from helpers import FULL_CORE_DOMAIN, GOLDEN, ROOT, make_input
 from oracle import pyoracle as po
 
 
@@ -66,7 +66,7 @@ def test_input_file_parsing():
     assert np.float32(inp.radial_ray_sep) == np.float32(0.5)
 
 
-@pytest.mark.parametrize("kw", [dict(), CONFIG2,
+@pytest.mark.parametrize("kw", [dict(), FULL_CORE_DOMAIN,
                                 dict(n_azimuthal=32, n_polar_angles=6, n_egroups=32, height=100.0,
                                      cai=5, decomp_assemblies_ax=4, radial_ray_sep=0.05)])
 def test_derived_shape_matches_oracle(kw):
@@ -85,8 +85,8 @@ def test_derived_shape_matches_oracle(kw):
     assert np.float32(inp.domain_height) == np.float32(ref["domain_height"])
 
 
-def test_config2_shape():
-    inp = make_input(**CONFIG2)
+def test_full_core_domain_shape():
+    inp = make_input(**FULL_CORE_DOMAIN)
     assert (inp.ntracks_2D, inp.z_stacked, inp.ntracks, inp.n_source_regions_per_node) == \
         (19386, 80, 15508800, 5000)
 
