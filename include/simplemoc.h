@@ -222,7 +222,8 @@ int moc_transport_sweep(moc_ctx *ctx, long *segments_processed);
  * values.  The device keeps b_psi from the first call on (zero, or NaN
  * after a non-finite normalisation factor, as the host copy would be).
  * Counts 2 x pieces for axial_exp 2 and 0 for axial_exp 0, as the
- * reference does.  Piece records take two slots per piece. */
+ * reference does.  The backward replay reads the forward piece records
+ * in reverse, so a two-way sweep needs no more record memory. */
 int moc_two_way_sweep(moc_ctx *ctx, long *segments_processed);
 /* src/comms.c:5-196 for one domain: no-op (reflective) or vacuum tally. */
 int moc_boundary_exchange(moc_ctx *ctx);

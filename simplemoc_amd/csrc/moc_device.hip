@@ -109,9 +109,9 @@ __global__ __launch_bounds__(256) void k_trace_tw(TraceArgs a)
         trace_stack_tw<WRITE ? TRACE_WRITE : TRACE_COUNT, KPT>(a, s, a.sweep_key, s_w);
 }
 
-// 256/LPT tracks per block
-template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
-__global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
+// 256/LPT tracks per block (REV: the backward replay of a two-way sweep)
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB, bool REV>
+__device__ __forceinline__ void attenuate_tracks(const AttArgs &a)
 {
     extern __shared__ float2 s_tab[];
     for (int e = threadIdx.x; e < a.N; e += blockDim.x)
@@ -126,8 +126,22 @@ __global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
     for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
         const long long t = a.t_begin + b * TPB + slot;
         if (t < a.t_end)
-            attenuate_track<LPT, GPL, QUAD, T64, EXPM, PB>(a, s_tab, t, l, lane_base);
+            attenuate_track<LPT, GPL, QUAD, T64, EXPM, PB, REV>(a, s_tab, t, l, lane_base);
     }
+}
+
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
+__global__ __launch_bounds__(256) void k_attenuate(AttArgs a)
+{
+    attenuate_tracks<LPT, GPL, QUAD, T64, EXPM, PB, false>(a);
+}
+
+// the replay holds a few more registers per piece in flight; with 2 pieces
+// per batch it keeps the forward kernel's occupancy without spilling
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM>
+__global__ __launch_bounds__(256) void k_attenuate_rev(AttArgs a)
+{
+    attenuate_tracks<LPT, GPL, QUAD, T64, EXPM, 2, true>(a);
 }
 
 // ---------------------------------------------------------------------
@@ -876,6 +890,7 @@ struct moc_ctx {
     float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
     double *d_cosd = nullptr, *d_sind = nullptr;
     float *d_pw = nullptr, *d_z = nullptr, *d_psi = nullptr, *d_bpsi = nullptr;
+    float *d_zb = nullptr;      // two-way sweeps: final forward heights
     bool psi_pending = false;   // psi still owes the last renormalisation factor
     // sources
     float *d_src = nullptr, *d_flux = nullptr, *d_sig = nullptr, *d_vol = nullptr;
@@ -957,7 +972,7 @@ static int dalloc(Tp **p, size_t count)
 static void free_all(moc_ctx *c)
 {
     void *ptrs[] = {c->d_segoff, c->d_seglen, c->d_az, c->d_mu, c->d_cosd, c->d_sind, c->d_pw,
-                    c->d_z, c->d_psi, c->d_bpsi, c->d_src, c->d_flux, c->d_sig,
+                    c->d_z, c->d_psi, c->d_bpsi, c->d_zb, c->d_src, c->d_flux, c->d_sig,
                     c->d_vol, c->d_mat, c->d_xs, c->d_scT, c->d_coef, c->d_acc, c->d_table,
                     c->d_cnt, c->d_off, c->d_stack_cnt, c->d_stack_off, c->d_rec,
                     c->d_stream, c->d_scan_tmp, c->d_partial,
@@ -1301,15 +1316,24 @@ static TraceArgs trace_args(moc_ctx *c)
 template <int LPT, int GPL, bool QUAD, bool T64, int EXPM>
 static void launch_att_pb(moc_ctx *c, const AttArgs &a, int grid, size_t lds)
 {
+    if (a.zrev) {
+        // the backward replay (2 pieces per batch)
+        hipLaunchKernelGGL((k_attenuate_rev<LPT, GPL, QUAD, T64, EXPM>), dim3(grid), dim3(256), lds,
+                           c->stream, a);
+        return;
+    }
     switch (c->att_pb) {
     case 1:
-        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 1>), dim3(grid), dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 1>), dim3(grid), dim3(256), lds,
+                           c->stream, a);
         break;
     case 2:
-        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 2>), dim3(grid), dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 2>), dim3(grid), dim3(256), lds,
+                           c->stream, a);
         break;
     default:
-        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 4>), dim3(grid), dim3(256), lds, c->stream, a);
+        hipLaunchKernelGGL((k_attenuate<LPT, GPL, QUAD, T64, EXPM, 4>), dim3(grid), dim3(256), lds,
+                           c->stream, a);
         break;
     }
 }
@@ -1457,15 +1481,13 @@ static int records_budget(moc_ctx *c, long long S, long long reserve, long long 
 
 // Count pass and offsets of the next sweep (z unchanged), then the chunks:
 // one when all records fit, else runs of whole stacks of at most the
-// budget's records each.  A two-way sweep keeps two records per piece
-// (forward, then backward at chunk offset S).
+// budget's records each.
 constexpr long long RECORD_RESERVE = 4LL << 30;
 
 static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
                       long long reserve = RECORD_RESERVE, bool two_way = false)
 {
     const long long nst = c->nt2 * c->np;
-    const long long per = two_way ? 2 : 1;
     TraceArgs ta = trace_args(c);
     launch_trace(c, ta, false, two_way);
     HIP_TRY(hipGetLastError());
@@ -1476,12 +1498,12 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
     HIP_TRY(hipMemcpyAsync(&S, c->d_off + c->T, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     long long budget = 0;
-    TRY(records_budget(c, per * S, reserve, &budget));
-    const long long pieces = budget / per;   // pieces one chunk may hold
+    TRY(records_budget(c, S, reserve, &budget));
+    const long long pieces = budget;   // pieces one chunk may hold
     chunks.clear();
     if (S <= pieces) {
         chunks.push_back({0, nst, 0, S});
-        TRY(ensure_records(c, per * S, budget));
+        TRY(ensure_records(c, S, budget));
     } else {
         std::vector<long long> sc(nst);
         HIP_TRY(hipMemcpy(sc.data(), c->d_stack_cnt, nst * 8, hipMemcpyDeviceToHost));
@@ -1501,15 +1523,15 @@ static int plan_trace(moc_ctx *c, std::vector<Chunk> &chunks, long long *S_out,
         biggest = std::max(biggest, acc);
         if (biggest > pieces && c->opt.max_records == 0)
             return moc_set_error(MOC_ERR_NOMEM, "one z-stack needs %lld piece records, more than "
-                                 "device memory holds", per * biggest);
+                                 "device memory holds", biggest);
         // exact fit: every chunk reuses the one buffer
-        if (per * biggest > c->rec_cap || c->rec_cap > budget) {
+        if (biggest > c->rec_cap || c->rec_cap > budget) {
             if (c->d_rec)
                 HIP_TRY(hipFree(c->d_rec));
             c->d_rec = nullptr;
             c->rec_cap = 0;
-            TRY(dalloc(&c->d_rec, per * biggest));
-            c->rec_cap = per * biggest;
+            TRY(dalloc(&c->d_rec, biggest));
+            c->rec_cap = biggest;
         }
     }
     if (c->opt.qsr_mode == MOC_QSR_LIBC) {
@@ -1534,9 +1556,8 @@ static int write_chunk(moc_ctx *c, const Chunk &ch, bool commit, const uint32_t 
     ta.rec = c->d_rec;
     ta.rec_cap = c->rec_cap;
     if (two_way) {
-        // forward records in [0, S), backward ones in [S, 2S)
         ta.rec_cap = ch.S;
-        ta.rec_b = c->d_rec + ch.S;
+        ta.zb = c->d_zb;
     }
     ta.rec_base = ch.rec_base;
     ta.s_begin = ch.s0;
@@ -1639,6 +1660,8 @@ __global__ void k_init_bpsi(float *__restrict__ x, long long n, const float *__r
 
 static int ensure_bpsi(moc_ctx *c)
 {
+    if (!c->d_zb)
+        TRY(dalloc(&c->d_zb, (size_t)c->T));
     if (c->d_bpsi)
         return MOC_OK;
     const long long n = c->T * c->G;
@@ -1684,11 +1707,17 @@ static int sweep(moc_ctx *c, bool two_way, long *segments_processed)
         launch_attenuate(c, fwd);
         HIP_TRY(hipGetLastError());
         if (two_way) {
+            // the same records, replayed backwards from each track's final
+            // forward height
             AttArgs bwd = fwd;
-            bwd.rec = c->d_rec + chunks[k].S;
             bwd.psi = c->d_bpsi;
             bwd.psi_scale = nullptr;   // renormalisation scales b_psi in place
             bwd.mu = c->d_mu + c->np;
+            bwd.zrev = c->d_zb;
+            bwd.fai = c->fai;
+            bwd.inv_fai = 1.0f / (float)c->fai;
+            bwd.dz_att = c->in.height / (float)(c->fai * c->decomp * c->cai);
+            bwd.err = c->d_err;
             launch_attenuate(c, bwd);
             HIP_TRY(hipGetLastError());
         }

@@ -65,10 +65,9 @@ struct TraceArgs {
     int *sort_id;
     long long key_base;
     // two-way sweeps: fine intervals per node, and where the write pass
-    // puts the backward records (track t's at the same offsets as its
-    // forward ones)
+    // leaves each track's final forward height (the backward replay's start)
     int nint;
-    float4 *rec_b;
+    float *zb;
 };
 
 // Attenuation: lanes of a track own groups g = lane + r*LPT.
@@ -95,6 +94,13 @@ struct AttArgs {
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch
     const int *cnt;             // pieces of track t (null: off[t+1] - off[t])
+    // backward replay of a two-way sweep (reverse attenuation): each track's
+    // starting height, and what its cells are computed with
+    const float *zrev;
+    int fai;
+    float inv_fai;              // 1.0f / fai
+    float dz_att;
+    unsigned *err;
 };
 
 }  // namespace moc

@@ -556,12 +556,10 @@ __device__ __forceinline__ TwSeg trace_segment_tw(float z, float s_full, bool po
 }
 
 // One z-stack of a two-way sweep (TRACE_COUNT or TRACE_WRITE).  The write
-// pass stores each track's forward records at off[t], then replays them
-// backwards into rec_b at the same offsets: piece q of the replay is
-// forward piece cnt-1-q with the region kept and the cell taken from the
-// replay height, which starts at the final forward height and drops by
-// ds*mu after every piece (:811-842).  With commit, z returns to the
-// track's starting height (:846-853).
+// pass stores each track's forward records at off[t] and its final forward
+// height in zb[t], where the backward replay (attenuate_track with REV)
+// starts; with commit, z returns to the track's starting height
+// (:846-853).
 template <int MODE, int KPT>
 __device__ __forceinline__ void trace_stack_tw(const TraceArgs &a, long long s, uint64_t skey, int *s_w)
 {
@@ -666,17 +664,9 @@ __device__ __forceinline__ void trace_stack_tw(const TraceArgs &a, long long s, 
             a.cnt[t0 + k] = cnt[r];
             continue;
         }
-        // backward replay records
-        EmitCtx eb = ec;
-        eb.rec = a.rec_b;
-        float zb = z[r];
-        for (int q = cnt[r] - 1; q >= 0; q--) {
-            const long long qf = base[r] + q;
-            const float4 f = qf < a.rec_cap ? a.rec[qf] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const uint32_t qsr = __float_as_uint(f.z) / (uint32_t)a.fai;
-            (void)store_record(&eb, base[r] + (cnt[r] - 1 - q), zb, f.x, qsr);
-            zb -= f.x * mu;
-        }
+        if ((long long)cnt[r] != a.off[t0 + k + 1] - a.off[t0 + k])
+            atomicOr(a.err, ERR_RECORD_CAP);
+        a.zb[t0 + k] = z[r];
         if (a.commit)
             a.z[t0 + k] = pos ? a.sep * (float)k : a.sep * (float)(k + 1);
     }
@@ -725,7 +715,13 @@ __device__ __forceinline__ float bcast(float x, int src_in_track, int lane_base)
 // Track t of the problem described by `a`, on the LPT lanes l = 0..LPT-1
 // starting at wave lane lane_base (all of them call it with the same t).
 // s_tab holds the problem's exponential table.
-template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB>
+//
+// REV is the backward replay of a two-way sweep (src/solver.c:811-842):
+// the track's forward records in reverse order, each piece's region kept
+// and its cell taken from the replay height, which starts at the track's
+// final forward height (zrev) and drops by ds * mu (forward mu, a float
+// product then a float subtraction) after every piece; a.mu holds -mu.
+template <int LPT, int GPL, bool QUAD, bool T64, int EXPM, int PB, bool REV = false>
 __device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *s_tab, long long t,
                                                 int l, int lane_base)
 {
@@ -751,19 +747,58 @@ __device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *
             psi[r] = a.psi[t * G + gcl[r]] * pscale;
         const long long q0 = a.off[t] - a.rec_base;
         const long long q1 = a.cnt ? q0 + a.cnt[t] : a.off[t + 1] - a.rec_base;
-        for (long long qc = q0; qc < q1; qc += LPT) {
-            const int nq = (int)min((long long)LPT, q1 - qc);
+        float zr = REV ? a.zrev[t] : 0.f;
+        const float mu_f = -mu;
+        for (long long qi = q0; qi < q1; qi += LPT) {
+            const int nq = (int)min((long long)LPT, q1 - qi);
+            // REV: runs of records from the track's end
+            const long long qc = REV ? q1 - (qi - q0) - nq : qi;
             const float4 mine = a.rec[qc + min(l, nq - 1)];
+            // REV: each lane turns its own record's FSR into the region
+            const uint32_t mine_qsr = REV ? __float_as_uint(mine.z) / (uint32_t)a.fai : 0u;
             for (int p = 0; p < nq; p += PB) {
                 float ds[PB], zin[PB];
                 uint32_t fsr[PB];
                 float4 cf[PB][GPL];
 #pragma unroll
                 for (int u = 0; u < PB; u++) {
-                    const int pu = min(p + u, nq - 1);
+                    const int pp = min(p + u, nq - 1);
+                    const int pu = REV ? nq - 1 - pp : pp;
                     ds[u] = bcast<LPT>(mine.x, pu, lane_base);
-                    zin[u] = bcast<LPT>(mine.y, pu, lane_base);
-                    fsr[u] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
+                    if (REV) {
+                        // the cell of the replay height, as attenuate_fluxes
+                        // computes it (src/solver.c:38-45)
+                        const uint32_t qsr =
+                            (uint32_t)__float_as_int(bcast<LPT>(__int_as_float((int)mine_qsr), pu, lane_base));
+                        const int cell = (int)(zr / a.dz_att);
+                        float zi = zr - a.dz_att * ((float)cell + 0.5f);
+                        // cell % fai: a negative cell (z below -dz, never
+                        // expected) leaves a remainder <= 0, which the
+                        // forward trace clamps to 0
+                        int fine = 0;
+                        if (cell >= 0 && cell < (1 << 24)) {
+                            const int qt = (int)((float)cell * a.inv_fai);
+                            fine = cell - qt * a.fai;
+                            fine += fine < 0 ? a.fai : (fine >= a.fai ? -a.fai : 0);
+                        } else if (cell >= 0) {
+                            fine = cell % a.fai;
+                        } else if (p + u < nq && l == 0) {
+                            atomicOr(a.err, ERR_FINE_RANGE);
+                        }
+                        if (QUAD) {
+                            if (fine == 0)
+                                zi -= a.dz_att;
+                            else if (fine == a.fai - 1)
+                                zi += a.dz_att;
+                        }
+                        zin[u] = zi;
+                        fsr[u] = qsr * (uint32_t)a.fai + (uint32_t)fine;
+                        if (p + u < nq)
+                            zr -= ds[u] * mu_f;
+                    } else {
+                        zin[u] = bcast<LPT>(mine.y, pu, lane_base);
+                        fsr[u] = __float_as_uint(bcast<LPT>(mine.z, pu, lane_base));
+                    }
 #pragma unroll
                     for (int r = 0; r < GPL; r++)
                         cf[u][r] = a.coef[(size_t)fsr[u] * G + gcl[r]];
