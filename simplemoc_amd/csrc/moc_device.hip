@@ -892,6 +892,7 @@ struct moc_ctx {
     float *d_pw = nullptr, *d_z = nullptr, *d_psi = nullptr, *d_bpsi = nullptr;
     float *d_zb = nullptr;      // two-way sweeps: final forward heights
     bool psi_pending = false;   // psi still owes the last renormalisation factor
+    bool bpsi_pending = false;  // so does b_psi
     // sources
     float *d_src = nullptr, *d_flux = nullptr, *d_sig = nullptr, *d_vol = nullptr;
     int *d_mat = nullptr;
@@ -1408,16 +1409,20 @@ static void launch_trace(moc_ctx *c, const TraceArgs &ta, bool write, bool two_w
 #undef MOC_TRACE_LAUNCH
 }
 
-// apply a pending renormalisation to psi in memory (before anything reads it)
+// apply a pending renormalisation to psi and b_psi in memory (before
+// anything else reads them, and before the next factor replaces it)
 static int materialize_psi(moc_ctx *c)
 {
-    if (!c->psi_pending)
-        return MOC_OK;
     const long long npsi = c->T * c->G;
-    hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_psi,
-                       npsi, c->d_scal);
+    if (c->psi_pending)
+        hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_psi,
+                           npsi, c->d_scal);
+    if (c->bpsi_pending && c->d_bpsi)
+        hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream, c->d_bpsi,
+                           npsi, c->d_scal);
     HIP_TRY(hipGetLastError());
     c->psi_pending = false;
+    c->bpsi_pending = false;
     return MOC_OK;
 }
 
@@ -1711,7 +1716,7 @@ static int sweep(moc_ctx *c, bool two_way, long *segments_processed)
             // forward height
             AttArgs bwd = fwd;
             bwd.psi = c->d_bpsi;
-            bwd.psi_scale = nullptr;   // renormalisation scales b_psi in place
+            bwd.psi_scale = c->bpsi_pending ? c->d_scal + SC_NF : nullptr;
             bwd.mu = c->d_mu + c->np;
             bwd.zrev = c->d_zb;
             bwd.fai = c->fai;
@@ -1730,6 +1735,8 @@ static int sweep(moc_ctx *c, bool two_way, long *segments_processed)
         ms_att += b;
     }
     c->psi_pending = false;
+    if (two_way)
+        c->bpsi_pending = false;
     c->chunks = (long)chunks.size();
     if (chunks.size() > 1)
         TRY(release_records(c));
@@ -1844,20 +1851,19 @@ extern "C" int moc_renormalize_flux(moc_ctx *c)
         return moc_set_error(MOC_ERR_INVALID, "moc_renormalize_flux: null context");
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    // psi and b_psi owe one factor at a time: an older one is settled
+    // before the new factor overwrites it
+    TRY(materialize_psi(c));
     hipLaunchKernelGGL(k_fission_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->stream, c->d_flux,
                        c->d_vol, c->d_mat, c->d_xs, c->nval, c->G, c->fai, c->d_partial);
     hipLaunchKernelGGL(k_finish_nf, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial,
                        RED_BLOCKS, c->d_scal);
     hipLaunchKernelGGL(k_scale_flux, dim3(launch_grid(c->nval, 256)), dim3(256), 0, c->stream,
                        c->d_flux, c->d_vol, c->d_scal, c->nval, c->G, c->fai);
-    const long long npsi = c->T * c->G;
-    // psi owes one factor at a time: settle an older one first
-    TRY(materialize_psi(c));
-    if (c->d_bpsi)
-        hipLaunchKernelGGL(k_scale_psi, dim3(launch_grid(npsi, 256)), dim3(256), 0, c->stream,
-                           c->d_bpsi, npsi, c->d_scal);
     HIP_TRY(hipGetLastError());
-    c->psi_pending = true;   // applied when the next sweep loads psi
+    // applied when the next sweep loads them (src/solver.c:1217-1226)
+    c->psi_pending = true;
+    c->bpsi_pending = c->d_bpsi != nullptr;
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[1]));
     float ms;
@@ -1925,7 +1931,7 @@ extern "C" int moc_download(moc_ctx *c, moc_problem *dst, unsigned what)
         dst->in.n_source_regions_per_node != c->nsr || dst->in.fai != c->fai)
         return moc_set_error(MOC_ERR_INVALID, "moc_download: problem shape differs from the context");
     HIP_TRY(hipSetDevice(c->device));
-    if (what & MOC_DL_PSI)
+    if (what & (MOC_DL_PSI | MOC_DL_B_PSI))
         TRY(materialize_psi(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if ((what & MOC_DL_FLUX) && dst->fine_flux)
@@ -1973,8 +1979,10 @@ extern "C" int moc_upload_state(moc_ctx *c, const moc_problem *src, unsigned wha
     }
     if ((what & MOC_DL_Z) && src->z_height)
         HIP_TRY(hipMemcpy(c->d_z, src->z_height, c->T * 4, hipMemcpyHostToDevice));
-    if ((what & MOC_DL_B_PSI) && src->b_psi && c->d_bpsi)
+    if ((what & MOC_DL_B_PSI) && src->b_psi && c->d_bpsi) {
         HIP_TRY(hipMemcpy(c->d_bpsi, src->b_psi, (size_t)c->T * c->G * 4, hipMemcpyHostToDevice));
+        c->bpsi_pending = false;
+    }
     if (what & MOC_DL_LEAKAGE)
         HIP_TRY(hipMemcpy(c->d_scal + SC_LEAK, &src->leakage, 4, hipMemcpyHostToDevice));
     return MOC_OK;
@@ -2088,6 +2096,7 @@ extern "C" int moc_get_track_bpsi(moc_ctx *c, long track, float *psi_out)
     if (!c || !psi_out || track < 0 || track >= c->T)
         return moc_set_error(MOC_ERR_INVALID, "moc_get_track_bpsi: bad argument");
     HIP_TRY(hipSetDevice(c->device));
+    TRY(materialize_psi(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!c->d_bpsi) {
         // never swept: zero, or NaN after a non-finite normalisation factor

@@ -190,3 +190,33 @@ def test_dropin_two_way_transport_sweep(case, golden_index, tmp_path):
         n, _ = flux_errors(d[key], g[key], G=G if key == "fine_flux" else None)
         print(f"{case} drop-in {key}: vs FP32 reference {n:.2e}")
         assert n <= REF_TOL, (key, n)
+
+
+def test_renormalisation_twice_and_psi_reads_between():
+    """Renormalising twice without a sweep between scales f_psi and b_psi by
+    both factors (the older pending one is settled before the newer one
+    replaces it); reads of either flux in between see the scaled values."""
+    inp = make_input(n_egroups=16, height=10.0)
+    prob = sm.Problem.generate(inp, seed=12, alloc_b_psi=True)
+    o = oracle_for(prob, seed=1)
+    with sm.Solver(prob) as s:
+        s.two_way_sweep()
+        s.renormalize_flux()
+        mid = s.track_psi(5, backward=True)
+        s.renormalize_flux()
+        s.transport_sweep()
+        s.renormalize_flux()
+        s.two_way_sweep()
+        st = s.state(b_psi=True)
+    o.two_way_sweep()
+    o.renormalize_flux()
+    mid_o = o["b_psi"][5 * 16:6 * 16].copy()
+    o.renormalize_flux()
+    o.transport_sweep()
+    o.renormalize_flux()
+    o.two_way_sweep()
+    n, _ = flux_errors(mid, mid_o)
+    assert n <= NEAR, n
+    for k in ("fine_flux", "f_psi", "b_psi"):
+        norm, _ = flux_errors(st[k], o[k], G=16 if k == "fine_flux" else None)
+        assert norm <= NEAR, (k, norm)
