@@ -30,8 +30,9 @@
  *   SIMPLEMOC_SEED=<n>        hash seed (default 1)
  *   SIMPLEMOC_DEVICE=<n>      HIP device (default 0)
  *   SIMPLEMOC_DROPIN_OPENMP=0 the reference objects were built with OPENMP=no
- *   SIMPLEMOC_DROPIN_SYNC_PSI=0  do not copy the angular flux back after
- *                             each call (saves a full PCIe transfer of psi)
+ *   SIMPLEMOC_DROPIN_SYNC_PSI=0  do not copy the angular fluxes (forward and
+ *                             backward) back after each call (saves full
+ *                             PCIe transfers of psi)
  * After every call the state the reference would have changed (scalar
  * flux, angular flux, z heights, sources, segments_processed) is copied
  * back into the caller's arrays, so the caller sees the reference's

@@ -257,7 +257,7 @@ void write_back(const moc_ref_Params *P, const moc_ref_Input *I, unsigned what)
 {
     DropIn &s = g_state;
     if (!s.sync_psi)
-        what &= ~(unsigned)MOC_DL_PSI;
+        what &= ~(unsigned)(MOC_DL_PSI | MOC_DL_B_PSI);
     if (moc_download(s.ctx, &s.prob, what) != MOC_OK)
         die("download");
     const int G = s.G, fai = s.fai, np = I->n_polar_angles, zs = I->z_stacked;
