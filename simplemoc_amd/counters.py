@@ -56,7 +56,7 @@ EVENT_SETS = {
         ("SQ_WAIT_ANY", "wave-cycles waiting for anything"),
         ("SQ_WAIT_INST_ANY", "wave-cycles waiting for any instruction issue"),
         ("SQ_WAIT_INST_LDS", "wave-cycles waiting for LDS instruction issue"),
-        ("SQ_ACTIVE_INST_VMEM", "wave-cycles issuing vector memory instructions"),
+        ("SQ_ACTIVE_INST_ANY", "wave-cycles issuing any instruction"),
         ("SQ_ACTIVE_INST_VALU", "wave-cycles issuing VALU instructions"),
     ]]),
     3: ("Stall percentage", "RESOURCE_STALLS:ANY, PAPI_TOT_CYC", [[
@@ -156,7 +156,7 @@ def summary(sid, v) -> list:
         w = v.get("SQ_WAVE_CYCLES", 0.0)
         return [f"{'Waiting (any):':30s} {pct(v.get('SQ_WAIT_ANY', 0.0), w):.2f}%",
                 f"{'Waiting for issue:':30s} {pct(v.get('SQ_WAIT_INST_ANY', 0.0), w):.2f}%",
-                f"{'Issuing vector memory:':30s} {pct(v.get('SQ_ACTIVE_INST_VMEM', 0.0), w):.2f}%",
+                f"{'Issuing any instruction:':30s} {pct(v.get('SQ_ACTIVE_INST_ANY', 0.0), w):.2f}%",
                 f"{'Issuing VALU:':30s} {pct(v.get('SQ_ACTIVE_INST_VALU', 0.0), w):.2f}%"]
     if sid == 3:
         return [f"GPU Stalled Wave Cycles: {pct(v.get('SQ_WAIT_ANY', 0.0), v.get('SQ_WAVE_CYCLES', 0.0)):.2f}%"]
