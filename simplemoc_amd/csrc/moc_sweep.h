@@ -240,7 +240,8 @@ __device__ __forceinline__ SegResult trace_segment(float zk, float s_full, int k
     return r;
 }
 
-// block-wide helpers (blockDim.x a multiple of 64, at most 1024)
+// block-wide helpers (blockDim.x a multiple of 64, at most 1024); a
+// one-wave block needs neither LDS nor a barrier
 __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -250,6 +251,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int *s_w, int *total)
         int y = __shfl_up(x, o, 64);
         if (lane >= o)
             x += y;
+    }
+    if (nw == 1) {
+        *total = __shfl(x, 63, 64);
+        return x - v;
     }
     if (lane == 63)
         s_w[w] = x;
@@ -271,6 +276,8 @@ __device__ __forceinline__ int block_min(int v, int *s_w)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         v = min(v, __shfl_xor(v, o, 64));
+    if (nw == 1)
+        return v;
     if (lane == 0)
         s_w[w] = v;
     __syncthreads();
