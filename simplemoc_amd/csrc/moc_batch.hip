@@ -25,6 +25,7 @@
 // device, so moc_batch_run can replay the iterations as a captured
 // hipGraph.  Kernels only, no memset nodes.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -141,10 +142,10 @@ template <int KPT>
 __global__ __launch_bounds__(256) void k_btrace(const TraceArgs *__restrict__ probs,
                                                  const int *__restrict__ stack_prob,
                                                  const long long *__restrict__ stack_first,
-                                                 long long nst, int commit)
+                                                 long long s_begin, long long s_end, int commit)
 {
     __shared__ int s_w[16];
-    for (long long s = blockIdx.x; s < nst; s += gridDim.x) {
+    for (long long s = s_begin + blockIdx.x; s < s_end; s += gridDim.x) {
         const int p = stack_prob[s];
         TraceArgs a = probs[p];
         a.commit = commit;
@@ -272,7 +273,10 @@ struct moc_batch {
     std::vector<BProb> pb;
     int nprob = 0;
     long long T_all = 0, nst_all = 0, nval_all = 0, psi_all = 0, rec_cap = 0;
-    int kpt = 1, trace_threads = 64, max_N = 0, cus = 256;
+    int kpt = 1, max_N = 0, cus = 256;
+    // stacks are numbered by the width of their problem's trace workgroup;
+    // one trace launch per width: {first stack, end, threads}
+    std::vector<std::array<long long, 3>> trace_classes;
     // device buffers
     long long *d_segoff = nullptr;
     float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
@@ -337,17 +341,21 @@ int check_err(moc_batch *b)
 
 void launch_btrace(moc_batch *b, int commit)
 {
-    const int grid = (int)std::min(b->nst_all, (long long)(1 << 20));
-#define MOC_BT(K)                                                                              \
-    hipLaunchKernelGGL((k_btrace<K>), dim3(grid), dim3(b->trace_threads), 0, b->stream,       \
-                       b->d_tr, b->d_stack_prob, b->d_stack_first, b->nst_all, commit)
-    switch (b->kpt) {
-    case 1: MOC_BT(1); break;
-    case 2: MOC_BT(2); break;
-    case 4: MOC_BT(4); break;
-    default: MOC_BT(8); break;
-    }
+    for (const auto &cl : b->trace_classes) {
+        const long long s0 = cl[0], s1 = cl[1];
+        const int threads = (int)cl[2];
+        const int grid = (int)std::min(s1 - s0, (long long)(1 << 20));
+#define MOC_BT(K)                                                                          \
+    hipLaunchKernelGGL((k_btrace<K>), dim3(grid), dim3(threads), 0, b->stream, b->d_tr,   \
+                       b->d_stack_prob, b->d_stack_first, s0, s1, commit)
+        switch (b->kpt) {
+        case 1: MOC_BT(1); break;
+        case 2: MOC_BT(2); break;
+        case 4: MOC_BT(4); break;
+        default: MOC_BT(8); break;
+        }
 #undef MOC_BT
+    }
 }
 
 void launch_batt(moc_batch *b, const BatchItem *items, int n_items)
@@ -578,7 +586,6 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         q.val_first = b->nval_all;
         q.sig_first = sig_all;
         q.table_first = tab_all;
-        q.stack_first = b->nst_all;
         n_seg2d_all += P->n_seg2d;
         segoff_all += q.nt2 + 1;
         nt2_all += q.nt2;
@@ -599,7 +606,28 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         return moc_set_error(MOC_ERR_INVALID, "moc_batch_create: z stacks of %d tracks (at most 2048)",
                              max_zs);
     b->kpt = trace_kpt(max_zs);
-    b->trace_threads = (int)(((max_zs + b->kpt - 1) / b->kpt + 63) / 64 * 64);
+    {
+        // number the stacks problem by problem in order of trace width, so
+        // that each width is one contiguous run traced by right-sized
+        // workgroups (a one-wave stack then needs no barrier)
+        auto width = [&](const BProb &q) { return (int)(((q.zs + b->kpt - 1) / b->kpt + 63) / 64 * 64); };
+        std::vector<int> order(n);
+        for (int p = 0; p < n; p++)
+            order[p] = p;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return width(b->pb[x]) < width(b->pb[y]); });
+        long long s_run = 0;
+        b->trace_classes.clear();
+        for (int p : order) {
+            BProb &q = b->pb[p];
+            const int w = width(q);
+            if (b->trace_classes.empty() || b->trace_classes.back()[2] != w)
+                b->trace_classes.push_back({s_run, s_run, (long long)w});
+            q.stack_first = s_run;
+            s_run += q.nst;
+            b->trace_classes.back()[1] = s_run;
+        }
+    }
 
     TRY(dalloc(&b->d_segoff, segoff_all));
     TRY(dalloc(&b->d_seglen, n_seg2d_all));
