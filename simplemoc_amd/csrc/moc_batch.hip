@@ -277,6 +277,10 @@ struct moc_batch {
     // stacks are numbered by the width of their problem's trace workgroup;
     // one trace launch per width: {first stack, end, threads}
     std::vector<std::array<long long, 3>> trace_classes;
+    // classes after the first run on side streams, forked from and joined
+    // back into the main stream (concurrent branches of the graph)
+    hipStream_t side[3] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {};
     // device buffers
     long long *d_segoff = nullptr;
     float *d_seglen = nullptr, *d_az = nullptr, *d_mu = nullptr;
@@ -321,6 +325,14 @@ void free_batch(moc_batch *b)
     for (auto &e : b->ev)
         if (e)
             (void)hipEventDestroy(e);
+    for (int c = 0; c < 3; c++) {
+        if (b->join_ev[c])
+            (void)hipEventDestroy(b->join_ev[c]);
+        if (b->side[c])
+            (void)hipStreamDestroy(b->side[c]);
+    }
+    if (b->fork_ev)
+        (void)hipEventDestroy(b->fork_ev);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
 }
@@ -339,15 +351,22 @@ int check_err(moc_batch *b)
     return MOC_OK;
 }
 
-void launch_btrace(moc_batch *b, int commit)
+int launch_btrace(moc_batch *b, int commit)
 {
-    for (const auto &cl : b->trace_classes) {
+    const int nc = (int)b->trace_classes.size();
+    if (nc > 1)
+        HIP_TRY(hipEventRecord(b->fork_ev, b->stream));
+    for (int c = 0; c < nc; c++) {
+        const auto &cl = b->trace_classes[c];
         const long long s0 = cl[0], s1 = cl[1];
         const int threads = (int)cl[2];
         const int grid = (int)std::min(s1 - s0, (long long)(1 << 20));
-#define MOC_BT(K)                                                                          \
-    hipLaunchKernelGGL((k_btrace<K>), dim3(grid), dim3(threads), 0, b->stream, b->d_tr,   \
-                       b->d_stack_prob, b->d_stack_first, s0, s1, commit)
+        hipStream_t st = c == 0 ? b->stream : b->side[c - 1];
+        if (c > 0)
+            HIP_TRY(hipStreamWaitEvent(st, b->fork_ev, 0));
+#define MOC_BT(K)                                                                                 \
+    hipLaunchKernelGGL((k_btrace<K>), dim3(grid), dim3(threads), 0, st, b->d_tr, b->d_stack_prob, \
+                       b->d_stack_first, s0, s1, commit)
         switch (b->kpt) {
         case 1: MOC_BT(1); break;
         case 2: MOC_BT(2); break;
@@ -355,7 +374,13 @@ void launch_btrace(moc_batch *b, int commit)
         default: MOC_BT(8); break;
         }
 #undef MOC_BT
+        HIP_TRY(hipGetLastError());
+        if (c > 0)
+            HIP_TRY(hipEventRecord(b->join_ev[c - 1], st));
     }
+    for (int c = 1; c < nc; c++)
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->join_ev[c - 1], 0));
+    return MOC_OK;
 }
 
 void launch_batt(moc_batch *b, const BatchItem *items, int n_items)
@@ -384,19 +409,20 @@ void launch_batt(moc_batch *b, const BatchItem *items, int n_items)
 }
 
 // one iteration: every problem swept once (no host interaction)
-void enqueue_iteration(moc_batch *b)
+int enqueue_iteration(moc_batch *b)
 {
     const bool t64 = b->opt.tally == MOC_TALLY_F64;
     if (t64)
         hipLaunchKernelGGL(k_bseed, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
                            b->d_flux, b->d_acc, b->nval_all);
-    launch_btrace(b, 1);
+    TRY(launch_btrace(b, 1));
     launch_batt(b, b->d_items, b->n_items);
     if (t64)
         hipLaunchKernelGGL(k_bfinal, dim3(launch_grid(b->nval_all, 256)), dim3(256), 0, b->stream,
                            b->d_acc, b->d_flux, b->nval_all);
     hipLaunchKernelGGL(k_bstep, dim3(launch_grid(b->nprob, 64)), dim3(64), 0, b->stream, b->d_sweep,
                        b->nprob);
+    return MOC_OK;
 }
 
 // Work items of the given problems: runs of tracks_per_item tracks (whole
@@ -627,6 +653,16 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
             s_run += q.nst;
             b->trace_classes.back()[1] = s_run;
         }
+        // widths are multiples of 64 up to 256 (trace_kpt), so at most four classes
+        const int nc = (int)b->trace_classes.size();
+        if (nc > 4)
+            return moc_set_error(MOC_ERR_STATE, "moc_batch_create: %d trace widths", nc);
+        if (nc > 1)
+            HIP_TRY(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+        for (int c = 1; c < nc; c++) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->side[c - 1], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->join_ev[c - 1], hipEventDisableTiming));
+        }
     }
 
     TRY(dalloc(&b->d_segoff, segoff_all));
@@ -793,8 +829,7 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
     // the first sweep's trace (z untouched): its piece counts order the
     // work queue, its records feed the tuner
     HIP_TRY(hipEventRecord(b->ev[2], b->stream));
-    launch_btrace(b, 0);
-    HIP_TRY(hipGetLastError());
+    TRY(launch_btrace(b, 0));
     std::vector<int> cnt(b->T_all + 1);
     HIP_TRY(hipMemcpyAsync(cnt.data(), b->d_cnt, (b->T_all + 1) * 4, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
@@ -870,9 +905,18 @@ extern "C" int moc_batch_run(moc_batch *b, int iterations)
         b->graph = nullptr;
         b->graph_iters = 0;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int it = 0; it < iterations; it++)
-            enqueue_iteration(b);
-        HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+        int rc = MOC_OK;
+        for (int it = 0; it < iterations && rc == MOC_OK; it++)
+            rc = enqueue_iteration(b);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (rc != MOC_OK) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(ec);
+        b->graph = g;
         HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
         b->graph_iters = iterations;
     }
@@ -883,7 +927,7 @@ extern "C" int moc_batch_run(moc_batch *b, int iterations)
         HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
     } else {
         for (int it = 0; it < iterations; it++)
-            enqueue_iteration(b);
+            TRY(enqueue_iteration(b));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
