@@ -1,0 +1,79 @@
+"""Deep z stacks on the GPU: stacks of 300, 600, 1500 and 2048 tracks (the
+largest the library accepts), so the trace runs with 2, 4 and 8 stacked
+tracks per thread and multi-wave workgroups, in the one-way sweep
+(src/solver.c:283-552), the two-way sweep (src/solver.c:556-891) and the
+batched sweep with shallow and deep problems in one batch.
+
+Tolerances as for the other parity tests: trace (segment counts, z
+heights) exact; GPU vs oracle in the same modes (hash draws, FP64 tally and
+reductions) within NEAR = 1e-6 normwise, k_eff within 1e-6.  Fine heights
+height / (cai * fai) are whole numbers, as the two-way comparison needs.
+"""
+import numpy as np
+import pytest
+
+import simplemoc_amd as sm
+from helpers import flux_errors, make_input, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+NEAR = 1e-6
+# z stack length -> (cai, fai) with a whole-number fine height
+DEEP = {300: (3, 5), 600: (2, 5), 1500: (3, 5), 2048: (4, 4)}
+
+
+def deep_problem(zs, G=8, two_way=False):
+    cai, fai = DEEP[zs]
+    inp = make_input(height=zs * 0.25, cai=cai, fai=fai, n_polar_angles=2, radial_ray_sep=2.0,
+                     n_egroups=G)
+    prob = sm.Problem.generate(inp, seed=zs + G, alloc_b_psi=two_way)
+    assert prob.dims["z_stacked"] == zs
+    return prob
+
+
+@pytest.mark.parametrize("two_way", [False, True])
+@pytest.mark.parametrize("zs", sorted(DEEP))
+def test_deep_stack_matches_oracle(zs, two_way):
+    print(f"deep stack zs={zs} two_way={two_way}", flush=True)
+    prob = deep_problem(zs, two_way=two_way)
+    opts = sm.Options()
+    o = oracle_for(prob, seed=opts.seed)
+    with sm.Solver(prob, opts) as s:
+        out_g = s.iterate(2, 1.0, two_way=two_way)
+        st = s.state(b_psi=two_way)
+    out_o = o.iterate(2, 1.0, two_way=two_way)
+    assert [x[0] for x in out_g] == [x[0] for x in out_o], "segment counts"
+    np.testing.assert_array_equal(st["z_height"], o["z_height"])
+    if two_way:
+        np.testing.assert_array_equal(st["z_height"], prob.array("z_height"))
+    for (_, kg, _), (_, ko, _) in zip(out_g, out_o):
+        assert abs(kg - ko) <= 1e-6 * max(1.0, abs(ko)), (kg, ko)
+    keys = ("fine_flux", "fine_source", "f_psi") + (("b_psi",) if two_way else ())
+    for k in keys:
+        norm, _ = flux_errors(st[k], o[k], G=8 if k in ("fine_flux", "fine_source") else None)
+        assert norm <= NEAR, (k, norm)
+
+
+def test_batch_with_deep_and_shallow_stacks():
+    """One batch of a 20-track, a 300-track and a 1500-track stack problem:
+    the batch traces with the deepest problem's tracks per thread and one
+    launch per workgroup width; every problem equals its single-problem
+    sweep (same hash draws, FP64 tally)."""
+    probs = [deep_problem(300, G=16), sm.Problem.generate(make_input(n_egroups=32), seed=5),
+             deep_problem(1500, G=8)]
+    seeds = [11, 12, 13]
+    iters = 3
+    with sm.Batch(probs, seeds, tally="f64") as b:
+        b.run(iters)
+        segs = b.segments()
+        got = [b.state(p) for p in range(len(probs))]
+    for p, prob in enumerate(probs):
+        print(f"batch problem {p}: zs={prob.dims['z_stacked']}", flush=True)
+        with sm.Solver(prob, sm.Options(seed=seeds[p], tally="f64")) as s:
+            tot = sum(s.transport_sweep() for _ in range(iters))
+            st = s.state()
+        assert segs[p] == tot
+        np.testing.assert_array_equal(got[p]["z_height"], st["z_height"])
+        np.testing.assert_array_equal(got[p]["f_psi"], st["f_psi"])
+        norm, _ = flux_errors(got[p]["fine_flux"], st["fine_flux"], G=prob.dims["G"])
+        assert norm <= NEAR, (p, norm)
