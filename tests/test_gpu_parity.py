@@ -46,11 +46,15 @@ def assert_close(out_g, st, out_o, o, G, tol=NEAR, ktol=1e-6):
         assert norm <= tol, (k, norm)
 
 
+# group counts from 1 to the largest supported (512: 8 groups per lane on
+# 64 lanes), and 1 or 2 fine intervals (the fit falls back to fewer rows)
 @pytest.mark.parametrize("G,fai,axial_exp,npol", [
     (8, 5, 2, 4), (16, 3, 2, 6), (32, 5, 2, 4), (64, 4, 2, 4), (128, 5, 2, 2), (100, 5, 2, 4),
-    (8, 3, 0, 4), (128, 5, 0, 2),
+    (8, 3, 0, 4), (128, 5, 0, 2), (1, 5, 2, 4), (5, 1, 2, 4), (12, 2, 2, 4), (200, 5, 2, 2),
+    (300, 2, 2, 2), (512, 5, 2, 2),
 ])
 def test_hash_mode_matches_oracle(G, fai, axial_exp, npol):
+    print(f"hash G={G} fai={fai} axial_exp={axial_exp} npol={npol}", flush=True)
     inp = make_input(n_egroups=G, fai=fai, axial_exp=axial_exp, n_polar_angles=npol)
     prob = sm.Problem.generate(inp, seed=G * 10 + fai)
     out_g, st, out_o, o = run_both(prob, 3)
