@@ -45,12 +45,14 @@ def write_track_file(path, per_azim, spacing=0.05, geometry="Geometry", cmfd=Fal
 
 
 def random_tracks(seed, n_azim=8, tracks_per_azim=(6, 14), mean_segments=20, width=21.42,
-                  n_regions=200, n_materials=8):
+                  n_regions=200, n_materials=8, empty_every=0):
     """Seeded straight-line tracks across a width x width square, each cut
     into segments whose lengths sum to the chord (per_azim for
-    write_track_file)."""
+    write_track_file); with empty_every = k, every k-th track (counting
+    from the first) has no segments."""
     rng = np.random.default_rng(seed)
     per_azim = []
+    n_made = 0
     for a in range(n_azim):
         phi = math.pi * (a + 0.5) / n_azim
         nt = int(rng.integers(tracks_per_azim[0], tracks_per_azim[1] + 1))
@@ -64,6 +66,9 @@ def random_tracks(seed, n_azim=8, tracks_per_azim=(6, 14), mean_segments=20, wid
             lens = np.diff(np.concatenate([[0.0], cuts, [chord]]))
             segs = [(float(L), int(rng.integers(n_materials)), int(rng.integers(n_regions)))
                     for L in lens]
+            if empty_every and n_made % empty_every == 0:
+                segs = []
+            n_made += 1
             ts.append((x0, y0, x1, y1, phi, segs))
         per_azim.append(ts)
     return per_azim

@@ -62,10 +62,11 @@ def test_loaded_tracks_match_reference_golden_run(case, golden_index, libc):
         assert nref <= REF_TOL, (key, nref)
 
 
-@pytest.mark.parametrize("two_way", [False, True])
-def test_loaded_tracks_hash_mode_matches_oracle(two_way, tmp_path):
+@pytest.mark.parametrize("two_way,empty_every", [(False, 0), (True, 0), (False, 4), (True, 4)])
+def test_loaded_tracks_hash_mode_matches_oracle(two_way, empty_every, tmp_path):
+    # empty_every = 4: every fourth 2D track has no segments
     path = tmp_path / "t.tracks"
-    write_random_track_file(path, seed=19, n_azim=12, mean_segments=30)
+    write_random_track_file(path, seed=19, n_azim=12, mean_segments=30, empty_every=empty_every)
     tf = sm.TrackFile(path)
     inp = make_input(n_egroups=32, height=10.0)
     inp.apply_track_file(tf)

@@ -117,14 +117,15 @@ def test_apply_track_file_to_input(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(po.REF_SERIAL),
                     reason="reference harness oracle/_ref/ref_serial not built")
-@pytest.mark.parametrize("two_way", [False, True])
-def test_loaded_tracks_match_reference_loader(two_way, tmp_path, libc):
+@pytest.mark.parametrize("two_way,empty_every", [(False, 0), (True, 0), (False, 3), (True, 3)])
+def test_loaded_tracks_match_reference_loader(two_way, empty_every, tmp_path, libc):
     """The reference's build_tracks with its loader and the product's
     reference-mode generator on the same file and seed give the same
     problem bit for bit; the oracle then reproduces the reference's
-    iterations on it bit for bit (one-way and two-way sweeps)."""
+    iterations on it bit for bit (one-way and two-way sweeps, and a file in
+    which every third 2D track has no segments)."""
     tpath = tmp_path / "t.tracks"
-    write_random_track_file(tpath, seed=11)
+    write_random_track_file(tpath, seed=11, empty_every=empty_every)
     d = dict(BASE, height=10.0)        # whole-number fine interval height
     ipath = tmp_path / "case.in"
     write_input(ipath, d)
