@@ -782,6 +782,7 @@ int create_batch(const moc_problem *const *probs, const uint64_t *seeds, int n,
         t.N = q.table_N;
         t.dx = q.table_dx;
         t.hdx = 0.5f * q.table_dx;
+        t.rdx = 1.0f / q.table_dx;
         t.maxv = q.table_maxv;
         t.cnt = b->d_cnt + q.track_first;
         t.t_end = q.T;

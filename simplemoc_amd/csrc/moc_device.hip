@@ -186,6 +186,7 @@ struct AdjArgs {
     int N;
     float dx;
     float hdx;
+    float rdx;                  // 1.0f / dx, correctly rounded
     float maxv;
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch
@@ -195,15 +196,15 @@ struct AdjArgs {
 };
 
 template <int EXPM>
-__device__ __forceinline__ float exp_term_d(const float2 *tab, float tau, float dx, float hdx,
-                                            float maxv, int N, float *dF)
+__device__ __forceinline__ float exp_term_d(const float2 *tab, float tau, float dx, float rdx,
+                                            float hdx, float maxv, int N, float *dF)
 {
     if (EXPM != MOC_EXP_TABLE) {
         const float F = -expm1f(-tau);
         *dF = 1.0f - F;
         return F;
     }
-    const float v = tau / dx + hdx;
+    const float v = div_by_recip(tau, dx, rdx) + hdx;   // as in exp_term
     const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
     const float2 sl = tab[iv];
     const bool sat = tau > maxv;
@@ -213,14 +214,14 @@ __device__ __forceinline__ float exp_term_d(const float2 *tab, float tau, float 
 
 // one piece forward: the psi update only
 template <bool QUAD, int EXPM>
-__device__ __forceinline__ void piece_forward(const float2 *tab, float dx, float hdx, float maxv,
-                                              int N, float4 c, float ds, float zin, float mu,
+__device__ __forceinline__ void piece_forward(const float2 *tab, float dx, float rdx, float hdx,
+                                              float maxv, int N, float4 c, float ds, float zin, float mu,
                                               float mu2, float &ps)
 {
     const float sig = c.w;
     const float tau = sig * ds;
     float dF;
-    const float F = exp_term_d<EXPM>(tab, tau, dx, hdx, maxv, N, &dF);
+    const float F = exp_term_d<EXPM>(tab, tau, dx, rdx, hdx, maxv, N, &dF);
     if (QUAD) {
         const float q0 = c.x + c.y * zin + c.z * zin * zin;
         const float q1 = c.y + 2.f * c.z * zin;
@@ -248,8 +249,8 @@ struct PieceGrad {
 // its powers instead of a division per term (a few ulp, far inside the
 // gradient tolerance).
 template <bool QUAD, int EXPM>
-__device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, float hdx,
-                                                   float maxv, int N, float4 c, float ds, float zin,
+__device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, float rdx,
+                                                   float hdx, float maxv, int N, float4 c, float ds, float zin,
                                                    float mu, float mu2, float psp, float bw,
                                                    float lam)
 {
@@ -257,7 +258,7 @@ __device__ __forceinline__ PieceGrad piece_reverse(const float2 *tab, float dx, 
     const float sig = c.w;
     const float tau = sig * ds;
     float dF;
-    const float F = exp_term_d<EXPM>(tab, tau, dx, hdx, maxv, N, &dF);
+    const float F = exp_term_d<EXPM>(tab, tau, dx, rdx, hdx, maxv, N, &dF);
     const float is = __builtin_amdgcn_rcpf(sig);
     o.I = 0.f;
     if (QUAD) {
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
         gg[r] = min(l + r * LPT, G - 1);   // clamped group for unconditional loads
         live[r] = l + r * LPT < G;
     }
-    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
+    const float dx = a.dx, rdx = a.rdx, hdx = a.hdx, maxv = a.maxv;
     const int N = a.N;
     const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
     float *tape = a.tape + ((long long)blockIdx.x * TPB + slot) * a.tape_stride;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
                         for (int r = 0; r < GPL; r++) {
                             if (live[r])
                                 tape[(long long)(c0 + u0 + k) * G + gg[r]] = ps[r];
-                            piece_forward<QUAD, EXPM>(s_tab, dx, hdx, maxv, N, c[k][r], ds[k],
+                            piece_forward<QUAD, EXPM>(s_tab, dx, rdx, hdx, maxv, N, c[k][r], ds[k],
                                                       zin[k], mu, mu2, ps[r]);
                         }
                     }
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(64 * ADJ_WAVES) void k_adjoint(AdjArgs a)
                         if (u1 - k >= 0) {
                             const float bw = wv_[k][r] * wt;
                             const PieceGrad o = piece_reverse<QUAD, EXPM>(
-                                s_tab, dx, hdx, maxv, N, c[k][r], ds[k], zin[k], mu, mu2,
+                                s_tab, dx, rdx, hdx, maxv, N, c[k][r], ds[k], zin[k], mu, mu2,
                                 psp[k][r], bw, lam[r]);
                             if (live[r]) {
                                 if (bw != 0.f)
@@ -1645,6 +1646,7 @@ static AttArgs att_args(moc_ctx *c, const Chunk &ch)
     aa.N = c->table_N;
     aa.dx = c->table_dx;
     aa.hdx = 0.5f * c->table_dx;
+    aa.rdx = 1.0f / c->table_dx;
     aa.maxv = c->table_maxv;
     aa.rec_base = ch.rec_base;
     aa.t_begin = ch.s0 * c->zs;
@@ -2283,6 +2285,7 @@ extern "C" int moc_adjoint(moc_ctx *c, const float *weights, double *J, double *
     aa.N = c->table_N;
     aa.dx = c->table_dx;
     aa.hdx = 0.5f * c->table_dx;
+    aa.rdx = 1.0f / c->table_dx;
     aa.maxv = c->table_maxv;
     const size_t lds = (size_t)c->table_N * sizeof(float2);
     float ms_k = 0.f;

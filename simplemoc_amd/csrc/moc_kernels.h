@@ -90,6 +90,7 @@ struct AttArgs {
     int N;
     float dx;
     float hdx;                  // 0.5f * dx
+    float rdx;                  // 1.0f / dx, correctly rounded
     float maxv;
     long long rec_base;         // record index of rec[0]
     long long t_begin, t_end;   // tracks of this launch

@@ -692,13 +692,38 @@ __device__ __forceinline__ void trace_stack_tw(const TraceArgs &a, long long s, 
 // The index clamp is branch-free: v <= 0 (and NaN) -> cell 0, which is the
 // reference's truncation for -1 < v < 0; v >= N-1 -> cell N-1 (never
 // reached below maxVal).
+//
+// tau / dx is divided with the reciprocal of dx (div_by_recip): the
+// quotient is the correctly rounded one for 2^-100 <= |tau| <= 2^100, and
+// outside that range its last bit cannot change the cell (tau / dx + hdx
+// rounds to hdx below it, and the cell is clamped or F saturates above it).
+
+// a / b correctly rounded, given y = 1/b correctly rounded (one IEEE
+// division for a divisor that many quotients share).  a*y can be 2 ulp
+// off; one correction through the fma remainder a - b*q makes the quotient
+// faithful, and a second is Markstein's final step, which turns a faithful
+// quotient and a correctly rounded reciprocal into the correctly rounded
+// quotient (P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Develop. 34(1), 1990; the
+// remainder a - b*q of a faithful q is exact).  Holds while a, the
+// quotient and the remainders stay normal.  5 VALU operations against 11,
+// one of them the quarter-rate reciprocal, for the IEEE division.
+__device__ __forceinline__ float div_by_recip(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r0 = fmaf(-b, q0, a);
+    const float q1 = fmaf(r0, y, q0);
+    const float r1 = fmaf(-b, q1, a);
+    return fmaf(r1, y, q1);
+}
+
 template <int EXPM>
-__device__ __forceinline__ float exp_term(const float2 *tab, float tau, float dx, float hdx,
-                                          float maxv, int N)
+__device__ __forceinline__ float exp_term(const float2 *tab, float tau, float dx, float rdx,
+                                          float hdx, float maxv, int N)
 {
     if (EXPM != MOC_EXP_TABLE)
         return -expm1f(-tau);
-    const float v = tau / dx + hdx;
+    const float v = div_by_recip(tau, dx, rdx) + hdx;
     const int iv = (int)fminf(fmaxf(v, 0.0f), (float)(N - 1));
     const float2 sl = tab[iv];
     const float F = sl.x * tau + sl.y;
@@ -733,7 +758,7 @@ __device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *
                                                 int l, int lane_base)
 {
     const int G = a.G;
-    const float dx = a.dx, hdx = a.hdx, maxv = a.maxv;
+    const float dx = a.dx, rdx = a.rdx, hdx = a.hdx, maxv = a.maxv;
     const int N = a.N;
     const float pscale = a.psi_scale ? *a.psi_scale : 1.0f;
     int gcl[GPL];
@@ -822,7 +847,7 @@ __device__ __forceinline__ void attenuate_track(const AttArgs &a, const float2 *
                         const float4 c = cf[u][r];
                         const float sig = c.w;
                         const float tau = sig * ds[u];
-                        const float F = exp_term<EXPM>(s_tab, tau, dx, hdx, maxv, N);
+                        const float F = exp_term<EXPM>(s_tab, tau, dx, rdx, hdx, maxv, N);
                         const float ps = psi[r];
                         float tally;
                         if (QUAD) {
