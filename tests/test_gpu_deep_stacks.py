@@ -54,16 +54,19 @@ def test_deep_stack_matches_oracle(zs, two_way):
         assert norm <= NEAR, (k, norm)
 
 
-def test_batch_with_deep_and_shallow_stacks():
-    """One batch of a 20-track, a 300-track and a 1500-track stack problem:
-    the batch traces with the deepest problem's tracks per thread and one
-    launch per workgroup width; every problem equals its single-problem
-    sweep (same hash draws, FP64 tally)."""
-    probs = [deep_problem(300, G=16), sm.Problem.generate(make_input(n_egroups=32), seed=5),
-             deep_problem(1500, G=8)]
-    seeds = [11, 12, 13]
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_with_deep_and_shallow_stacks(use_graph):
+    """One batch of z stacks of 600, 20, 1500 and 2048 tracks: the batch
+    traces with the deepest problem's 8 tracks per thread, so the four
+    problems need trace workgroups of 128, 64, 192 and 256 threads, one
+    launch per width (three of them on side streams joined back before the
+    attenuation), replayed as a graph or run eagerly.  Every problem equals
+    its single-problem sweep (same hash draws, FP64 tally)."""
+    probs = [deep_problem(600, G=16), sm.Problem.generate(make_input(n_egroups=32), seed=5),
+             deep_problem(1500, G=8), deep_problem(2048, G=8)]
+    seeds = [11, 12, 13, 14]
     iters = 3
-    with sm.Batch(probs, seeds, tally="f64") as b:
+    with sm.Batch(probs, seeds, tally="f64", use_graph=use_graph) as b:
         b.run(iters)
         segs = b.segments()
         got = [b.state(p) for p in range(len(probs))]
