@@ -24,7 +24,10 @@
  *   src/solver.c:1441-1464  interpolateTable
  *   src/utils.c:29-45       pairwise_sum
  *   src/utils.c:48-78       buildExponentialTable
- *   src/comms.c:5-196       fast_transfer_boundary_fluxes (border-rank part)
+ *   src/comms.c:5-196       fast_transfer_boundary_fluxes (border-rank part;
+ *                           no MPI run of the reference exists to compare
+ *                           with: pinned by the message layout's invariants,
+ *                           tests/test_oracle_vacuum.py)
  */
 #ifndef MOC_ORACLE_H
 #define MOC_ORACLE_H
