@@ -273,7 +273,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tally", default="f32", choices=["f64", "f32"])
     ap.add_argument("--single-mode", action="store_true",
-                    help="skip the short run of the other tally mode")
+                    help="only the main tally mode's plain iterations: skip the short run of "
+                         "the other tally mode and the host-buffer leg")
     ap.add_argument("--lanes-per-track", type=int, default=0)
     ap.add_argument("--pieces-per-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -282,6 +283,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched/plain leg")
     ap.add_argument("--no-two-way", action="store_true", help="skip the two-way sweep leg")
+    ap.add_argument("--no-host-buffers", action="store_true",
+                    help="skip the leg with the drop-in's per-call host copies")
     ap.add_argument("--large", action="store_true",
                     help="only the large shape (~200 GB of HBM): one iteration and one adjoint")
     ap.add_argument("--only", choices=["adjoint", "batched", "two_way"],
@@ -374,6 +377,46 @@ def main():
                 "roofline_frac": r["achieved"] / HBM_PEAK_GBS, "keff": r["keff"],
                 "sweep_ms_vs_plain": (r["sweep_ms"] / plain_sweep_ms) if plain_sweep_ms else None}
 
+    def host_buffers_leg(tally):
+        """Whole iterations with the host transfers the drop-in symbols make
+        on every call (simplemoc_amd/csrc/moc_dropin.cpp: flux, psi and z
+        copied back after transport_sweep, flux and psi after
+        renormalize_flux, the sources after update_sources) into pageable
+        host arrays: the PCIe-inclusive rate, with psi and without it
+        (SIMPLEMOC_DROPIN_SYNC_PSI=0).  The drop-in's further host copy from
+        these arrays into the reference's per-track arrays is not counted."""
+        nv = inp.n_source_regions_per_node * inp.fai * G
+        host = dict(fine_flux=np.zeros(nv, np.float32), fine_source=np.zeros(nv, np.float32),
+                    z_height=np.zeros(T, np.float32), f_psi=np.zeros(T * G, np.float32))
+        out = {"what": "plain iterations with the drop-in's per-call copies to pageable host "
+                       "arrays (flux, psi, z after the sweep; flux, psi after renormalisation; "
+                       "sources after the source update)"}
+        with sm.Solver(prob, sm.Options(tally=tally)) as solver:
+            keff = 1.0
+            for sync_psi in (True, False):
+                psi_bit = sm.DL_PSI if sync_psi else 0
+
+                def step(keff):
+                    segs = solver.transport_sweep()
+                    solver.download_arrays(host, sm.DL_FLUX | psi_bit | sm.DL_Z)
+                    solver.renormalize_flux()
+                    solver.download_arrays(host, sm.DL_FLUX | psi_bit)
+                    solver.update_sources(keff)
+                    solver.download_arrays(host, sm.DL_SOURCE)
+                    return segs, solver.compute_keff()
+
+                segs, keff = step(keff)   # warm-up
+                t0 = time.perf_counter()
+                segs, keff = step(keff)
+                dt = time.perf_counter() - t0
+                key = "with_psi" if sync_psi else "without_psi"
+                out[f"integrations_per_s_{key}"] = segs * G / dt
+                out[f"iteration_ms_{key}"] = dt * 1e3
+                log(f"[host buffers, psi {'synced' if sync_psi else 'not synced'}] iteration "
+                    f"{dt * 1e3:.1f} ms, {segs * G / dt:.3e} integrations/s")
+        out["psi_bytes_per_copy"] = T * G * 4
+        return out
+
     if args.only == "two_way":
         print(json.dumps({"metric": METRIC, "variant": "two_way", "n_gpus": 1,
                           "two_way": two_way_leg()}), flush=True)
@@ -427,6 +470,11 @@ def main():
             line["config"]["two_way"] = two_way_leg(main_run["sweep_ms"])
         except Exception as e:  # reported, never fatal to the plain-sweep measurement
             log(f"two-way leg failed: {e}")
+    if not (args.no_host_buffers or args.single_mode):
+        try:
+            line["config"]["host_buffers"] = host_buffers_leg(args.tally)
+        except Exception as e:  # reported, never fatal to the plain-sweep measurement
+            log(f"host-buffer leg failed: {e}")
     if not args.no_adjoint:
         try:
             line["config"]["adjoint"] = adjoint_leg(sm)

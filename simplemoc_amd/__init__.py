@@ -532,6 +532,28 @@ class Solver:
         _check(lib().moc_download(self._h, p._p, what))
         return p
 
+    _DL_KEYS = (("fine_flux", DL_FLUX), ("fine_source", DL_SOURCE), ("z_height", DL_Z),
+                ("f_psi", DL_PSI), ("b_psi", DL_B_PSI))
+
+    def download_arrays(self, arrays: dict, what: int) -> float:
+        """Copy device state into caller-owned float32 arrays (keys
+        fine_flux, fine_source, z_height, f_psi, b_psi; one per DL_* bit in
+        `what`, sized as in state()); returns the leakage."""
+        p = _Problem()
+        p.inp = I = self.problem._p.contents.inp
+        T, G = I.ntracks, I.n_egroups
+        nv = I.n_source_regions_per_node * I.fai * G
+        size = dict(fine_flux=nv, fine_source=nv, z_height=T, f_psi=T * G, b_psi=T * G)
+        for k, bit in self._DL_KEYS:
+            if what & bit:
+                a = arrays[k]
+                if a.dtype != np.float32 or not a.flags.c_contiguous or a.size != size[k]:
+                    raise MOCError(f"download_arrays: {k} must be a contiguous float32 array "
+                                   f"of {size[k]} values")
+                setattr(p, k, a.ctypes.data_as(_fp))
+        _check(lib().moc_download(self._h, C.pointer(p), what))
+        return p.leakage
+
     def state(self, psi: bool = True, b_psi: bool = False):
         """Fresh numpy copies of flux, source, (psi, b_psi,) z and leakage."""
         I = self.problem._p.contents.inp
@@ -546,12 +568,7 @@ class Solver:
         if b_psi:
             arrs["b_psi"] = np.zeros(T * G, np.float32)
             what |= DL_B_PSI
-        p = _Problem()
-        p.inp = I
-        for k, a in arrs.items():
-            setattr(p, k, a.ctypes.data_as(_fp))
-        _check(lib().moc_download(self._h, C.pointer(p), what))
-        arrs["leakage"] = p.leakage
+        arrs["leakage"] = self.download_arrays(arrs, what)
         return arrs
 
     def timing(self) -> dict:
