@@ -87,6 +87,29 @@ def test_adjoint_several_groups_per_lane(G):
     assert np.abs(gc - gc64).max() <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("axial_z_sep", [0.25, 0.1, 0.05])
+def test_adjoint_deep_stacks(axial_z_sep):
+    """z stacks of 400, 1000 and 2000 tracks, whose trace runs 2, 4 and 8
+    stacked tracks per thread: the gradient against the oracle's FP64
+    reverse-mode gradient, as above."""
+    prob, solver, w, _ = build("table", 2, seed=5, axial_z_sep=axial_z_sep, decomp_assemblies_ax=1,
+                               radial_ray_sep=8.0, n_2D_source_regions_per_assembly=100,
+                               n_azimuthal=8, n_polar_angles=2, n_egroups=16)
+    zs = prob.dims["z_stacked"]
+    assert zs == {0.25: 400, 0.1: 1000, 0.05: 2000}[axial_z_sep]
+    print(f"adjoint, z stacks of {zs} tracks", flush=True)
+    with solver:
+        J, gs, gc = solver.adjoint(w)
+        coef, sigma = solver.coefficients()
+    o = oracle_for(prob)
+    J64, gs64, gc64 = o.adjoint_f64(coef.astype(np.float64), sigma.astype(np.float64),
+                                    w.astype(np.float64), exp_mode=po.EXP_TABLE)
+    assert J == pytest.approx(J64, rel=1e-5)
+    scale = max(np.abs(gs64).max(), np.abs(gc64).max())
+    assert np.abs(gs - gs64).max() <= 1e-5 * scale
+    assert np.abs(gc - gc64).max() <= 1e-5 * scale
+
+
 @pytest.mark.parametrize("grad_tally", ["f64", "f32"])
 def test_adjoint_matches_central_differences(grad_tally):
     prob, solver, w, rng = build("intrinsic", 2, seed=7, grad_tally=grad_tally)
