@@ -1,8 +1,9 @@
 """Deep z stacks on the GPU: stacks of 300, 600, 1500 and 2048 tracks (the
 largest the library accepts), so the trace runs with 2, 4 and 8 stacked
 tracks per thread and multi-wave workgroups, in the one-way sweep
-(src/solver.c:283-552), the two-way sweep (src/solver.c:556-891) and the
-batched sweep with shallow and deep problems in one batch.
+(src/solver.c:283-552) and the two-way sweep (src/solver.c:556-891), with
+hash draws and with the host's rand() draws in the reference's serial
+order, and the batched sweep with shallow and deep problems in one batch.
 
 Tolerances as for the other parity tests: trace (segment counts, z
 heights) exact; GPU vs oracle in the same modes (hash draws, FP64 tally and
@@ -14,6 +15,7 @@ import pytest
 
 import simplemoc_amd as sm
 from helpers import flux_errors, make_input, oracle_for
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +53,33 @@ def test_deep_stack_matches_oracle(zs, two_way):
     keys = ("fine_flux", "fine_source", "f_psi") + (("b_psi",) if two_way else ())
     for k in keys:
         norm, _ = flux_errors(st[k], o[k], G=8 if k in ("fine_flux", "fine_source") else None)
+        assert norm <= NEAR, (k, norm)
+
+
+@pytest.mark.parametrize("two_way", [False, True])
+@pytest.mark.parametrize("zs", [300, 1500])
+def test_deep_stack_libc_draws_match_oracle(zs, two_way, libc):
+    """The host's rand() draws in the reference's serial order (stack by
+    stack, then the trace order inside a stack), consumed by a trace that
+    holds several stacked tracks per thread: GPU and oracle seeded alike
+    give the same pieces, z heights and (FP64 tally) fluxes."""
+    print(f"deep stack libc draws zs={zs} two_way={two_way}", flush=True)
+    prob = deep_problem(zs, two_way=two_way)
+    seed = 4242 + zs
+    libc.srand(seed)
+    with sm.Solver(prob, sm.Options(qsr="libc")) as s:
+        out_g = s.iterate(2, 1.0, two_way=two_way)
+        st = s.state(b_psi=two_way)
+    o = oracle_for(prob, qsr_mode=po.QSR_LIBC)
+    libc.srand(seed)
+    out_o = o.iterate(2, 1.0, two_way=two_way)
+    assert [x[0] for x in out_g] == [x[0] for x in out_o], "segment counts"
+    np.testing.assert_array_equal(st["z_height"], o["z_height"])
+    for (_, kg, _), (_, ko, _) in zip(out_g, out_o):
+        assert abs(kg - ko) <= 1e-6 * max(1.0, abs(ko)), (kg, ko)
+    keys = ("fine_flux", "f_psi") + (("b_psi",) if two_way else ())
+    for k in keys:
+        norm, _ = flux_errors(st[k], o[k], G=8 if k == "fine_flux" else None)
         assert norm <= NEAR, (k, norm)
 
 
