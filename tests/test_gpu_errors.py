@@ -1,0 +1,77 @@
+"""Argument checks of the C ABI with a GPU present.  Every case is rejected
+on the host with MOC_ERR_INVALID (raised as MOCError) before any kernel is
+launched, and a context that already exists stays usable afterwards.
+Limits: z stacks of at most 2048 tracks, at most 512 groups (8 per lane on
+64 lanes; narrower tracks take one group per lane), the adjoint up to 256
+groups, and the launch options' documented values.
+"""
+import numpy as np
+import pytest
+
+import simplemoc_amd as sm
+from helpers import make_input
+
+pytestmark = pytest.mark.gpu
+
+
+def small(**kw):
+    inp = make_input(**kw)
+    return sm.Problem.generate(inp, seed=1)
+
+
+def deep_2052():
+    # 513 cm / 0.25 cm = 2052 stacked tracks
+    prob = small(height=513.0, radial_ray_sep=4.0, n_polar_angles=2)
+    assert prob.dims["z_stacked"] > 2048
+    return prob
+
+
+def test_z_stack_deeper_than_supported():
+    with pytest.raises(sm.MOCError, match="z stacks"):
+        sm.Solver(deep_2052())
+
+
+def test_more_groups_than_supported():
+    with pytest.raises(sm.MOCError, match="groups"):
+        sm.Solver(small(n_egroups=513, radial_ray_sep=4.0, n_polar_angles=2))
+
+
+@pytest.mark.parametrize("kw,match", [(dict(lanes_per_track=12), "lanes_per_track"),
+                                      (dict(pieces_per_batch=3), "pieces_per_batch")])
+def test_bad_launch_options(kw, match):
+    with pytest.raises(sm.MOCError, match=match):
+        sm.Solver(small(), sm.Options(**kw))
+
+
+def test_several_groups_per_lane_need_64_lanes():
+    # 40 groups on 16 lanes would need 3 groups per lane
+    with pytest.raises(sm.MOCError, match="groups per lane"):
+        sm.Solver(small(n_egroups=40), sm.Options(lanes_per_track=16))
+
+
+def test_adjoint_group_limit_leaves_context_usable():
+    prob = small(n_egroups=300, radial_ray_sep=4.0, n_polar_angles=2)
+    d = prob.dims
+    with sm.Solver(prob) as s:
+        w = np.zeros(d["nsr"] * d["fai"] * d["G"], np.float32)
+        with pytest.raises(sm.MOCError, match="256 energy groups"):
+            s.adjoint(w)
+        assert s.transport_sweep() > 0
+
+
+def test_bad_track_index_and_array_shape():
+    prob = small()
+    with sm.Solver(prob) as s:
+        T = prob.dims["ntracks"]
+        for t in (-1, T):
+            with pytest.raises(sm.MOCError, match="bad argument"):
+                s.track_psi(t)
+        with pytest.raises(sm.MOCError, match="contiguous float32"):
+            s.download_arrays({"fine_flux": np.zeros(3, np.float32)}, sm.DL_FLUX)
+        assert s.transport_sweep() > 0
+        assert s.track_psi(T - 1).shape == (prob.dims["G"],)
+
+
+def test_batch_rejects_deep_stack():
+    with pytest.raises(sm.MOCError, match="z stacks"):
+        sm.Batch([small(), deep_2052()], [1, 2])
